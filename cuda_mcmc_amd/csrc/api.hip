@@ -1,0 +1,1002 @@
+// Host side of the C ABI (include/mhre.h).
+//
+// Replaces the reference's host layer src/mcmcjob.cu / src/mcmcjob.cuh
+// (CCalcModel / CMcmcFit: per-call cudaMalloc + HtoD of every array, the
+// whole object copied to the device, exit(1) on errors) with device-resident
+// sessions:
+//   * every array of every fit lives in one device arena (one hipMalloc);
+//   * the native ring+Gaussian kernels get their own packed inputs
+//     (float4 visibilities, closure triplets sorted into LDS-ring windows);
+//   * one iteration = one step launch per model kind + one exchange launch,
+//     driven by a device iteration clock so a cycle can be captured once in a
+//     hipGraph and replayed;
+//   * errors are returned as codes with a message (mhre_last_error).
+#include <hip/hip_runtime.h>
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#define MHRE_NO_MODEL_MACROS 1
+#include "ccalcmodel.h"
+#include "fitdesc.h"
+#include "engine.h"
+#include "mhre.h"
+
+using namespace mhre;
+
+namespace {
+
+thread_local std::string g_err;
+
+void set_err(const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_err = buf;
+}
+
+#define FAIL(code, ...)  \
+  do {                   \
+    set_err(__VA_ARGS__); \
+    return code;         \
+  } while (0)
+
+#define HIPCHK(x)                                                                        \
+  do {                                                                                   \
+    hipError_t e_ = (x);                                                                 \
+    if (e_ != hipSuccess) {                                                              \
+      set_err("%s failed: %s (%s:%d)", #x, hipGetErrorString(e_), __FILE__, __LINE__);   \
+      return e_ == hipErrorOutOfMemory ? MHRE_ENOMEM : MHRE_EHIP;                        \
+    }                                                                                    \
+  } while (0)
+
+#ifdef MHRE_NATIVE_EHT
+constexpr int kNativeEht = 1;
+#else
+constexpr int kNativeEht = 0;
+#endif
+
+bool env_flag(const char* name) {
+  const char* v = getenv(name);
+  return v && *v && strcmp(v, "0") != 0;
+}
+
+// ----------------------------------------------------------------- arena
+
+struct Arena {
+  size_t bytes = 0;
+  template <class T>
+  size_t reserve(size_t n) {
+    const size_t off = bytes;
+    bytes += ((n * sizeof(T) + 255) / 256) * 256;
+    if (n == 0) bytes += 256;
+    return off;
+  }
+};
+
+// ------------------------------------------------------------------ fits
+
+enum Kind { K_SGRA9 = 0, K_SGRA13 = 1, K_USER = 2, K_QUAD = 3 };
+
+struct FitHost {
+  MhreFitSpec s;
+  int kind = K_USER;
+  int nchain = 0;
+  int nvis_real = 0, ncph = 0, cpext = 0, nvis_k = 0, ntile = 0, ring_ok = 1;
+  int quad_n = 0;
+  // packed host inputs of the native kernels
+  std::vector<float> vis;        // 4 per visibility
+  std::vector<int> clo;          // 4 per closure
+  std::vector<float> clow;       // 4 per closure
+  std::vector<int> clo_range;
+  std::vector<float> qcoef;
+  std::vector<int> sumh;
+  // arena offsets
+  size_t o_vis, o_clo, o_clow, o_crange, o_qcoef;
+  size_t o_coor, o_dat, o_std2r, o_icoor, o_idat, o_pdescr, o_ivar, o_invar, o_pmint, o_pmaxt, o_beta;
+  size_t o_pcur, o_chi2c, o_pstp, o_ncnt, o_nacpt, o_nhist, o_nexch, o_sumh, o_tcur, o_flag, o_ptent, o_ptentn;
+  size_t o_pout, o_tout, o_chi2, o_datm, o_chi2m, o_mc;
+  bool has_datm = false;
+  FitDesc d{};
+};
+
+int validate(const MhreFitSpec& s, int idx) {
+  if (s.nbeta <= 0 || s.nseq <= 0 || s.nptot <= 0 || s.nprm <= 0 || s.ndat <= 0)
+    FAIL(MHRE_EINVAL, "fit %d: nbeta, nseq, nptot, nprm and ndat must be positive", idx);
+  if (s.nprm > s.nptot) FAIL(MHRE_EINVAL, "fit %d: nprm (%d) > nptot (%d)", idx, s.nprm, s.nptot);
+  if (s.nptot > MAXP) FAIL(MHRE_EINVAL, "fit %d: nptot %d exceeds %d", idx, s.nptot, MAXP);
+  if (s.nadj <= 0) FAIL(MHRE_EINVAL, "fit %d: nadj must be positive", idx);
+  if (s.nburn < 0 || s.niter < 0) FAIL(MHRE_EINVAL, "fit %d: negative nburn/niter", idx);
+  if (!s.pdescr || !s.ivar || !s.invar || !s.pmint || !s.pmaxt || !s.beta || !s.pcur || !s.chi2c ||
+      !s.pstp || !s.n_cnt || !s.n_acpt || !s.n_hist || !s.n_exch || !s.tcur || !s.flag || !s.ptent ||
+      !s.ptentn || !s.dat || !s.std2r)
+    FAIL(MHRE_EINVAL, "fit %d: a required array pointer is NULL", idx);
+  if (s.niter > 0 && (!s.pout || !s.tout || !s.chi2)) FAIL(MHRE_EINVAL, "fit %d: pout/tout/chi2 NULL", idx);
+  int nv = 0;
+  for (int i = 0; i < s.nptot; ++i) {
+    if (s.pdescr[i] != 0) {
+      if (s.invar[i] != nv) FAIL(MHRE_EINVAL, "fit %d: invar[%d] = %d, expected %d", idx, i, s.invar[i], nv);
+      if (nv < s.nprm && s.ivar[nv] != i) FAIL(MHRE_EINVAL, "fit %d: ivar[%d] = %d, expected %d", idx, nv, s.ivar[nv], i);
+      ++nv;
+    } else if (s.invar[i] != -1) {
+      FAIL(MHRE_EINVAL, "fit %d: invar[%d] must be -1 for a fixed parameter", idx, i);
+    }
+  }
+  if (nv != s.nprm) FAIL(MHRE_EINVAL, "fit %d: %d nonzero pdescr entries but nprm = %d", idx, nv, s.nprm);
+  const size_t pc = (size_t)s.nprm * s.nbeta * s.nseq;
+  for (size_t q = 0; q < pc; ++q)
+    if (s.n_cnt[q] < 0) FAIL(MHRE_EINVAL, "fit %d: n_cnt must be non-negative (reference: ones)", idx);
+  for (size_t q = 0; q < (size_t)s.nadj * pc; ++q)
+    if (s.n_hist[q] != 0 && s.n_hist[q] != 1) FAIL(MHRE_EINVAL, "fit %d: n_hist entries must be 0 or 1", idx);
+  for (int b = 0; b < s.nbeta; ++b)
+    if (!(s.beta[b] >= 0.0f)) FAIL(MHRE_EINVAL, "fit %d: beta[%d] must be >= 0", idx, b);
+  for (int i = 0; i < s.nptot; ++i)
+    if (s.pdescr[i] != 0 && !(s.pmint[i] <= s.pmaxt[i])) FAIL(MHRE_EINVAL, "fit %d: pmint[%d] > pmaxt[%d]", idx, i, i);
+  return MHRE_OK;
+}
+
+// Native EHT inputs from the reference arrays (src/imgpu.py:759-831 layout):
+// coor = [u0,v0,...] (+ [u1,v1,u2,v2] per closure when cpExt = 1),
+// dat/std2r = [amplitudes, closures], icoor = closure triplets, idat = [nvis, nchi2m, cpExt].
+int pack_eht(FitHost& h, int idx) {
+  const MhreFitSpec& s = h.s;
+  const int need = h.kind == K_SGRA9 ? 9 : 13;
+  if (s.nptot < need) FAIL(MHRE_EINVAL, "fit %d: model needs %d parameters, nptot = %d", idx, need, s.nptot);
+  if (!s.idat || s.nidat < 1 || !s.coor) FAIL(MHRE_EINVAL, "fit %d: idat/coor missing", idx);
+  const int nvis = s.idat[0];
+  const int cpext = s.nidat >= 3 ? s.idat[2] : 0;
+  if (nvis <= 0 || nvis > s.ndat) FAIL(MHRE_EINVAL, "fit %d: idat[0] = %d not in [1, ndat]", idx, nvis);
+  const int ncph = s.ndat - nvis;
+  if (s.ncoor < 2 * nvis) FAIL(MHRE_EINVAL, "fit %d: ncoor %d < 2*nvis", idx, s.ncoor);
+  if (ncph > 0 && cpext == 0 && (!s.icoor || s.nicoor < 3 * ncph))
+    FAIL(MHRE_EINVAL, "fit %d: nicoor %d < 3*ncph", idx, s.nicoor);
+  if (ncph > 0 && cpext != 0 && s.ncoor < 2 * nvis + 4 * ncph)
+    FAIL(MHRE_EINVAL, "fit %d: cpExt=1 needs ncoor >= 2*nvis + 4*ncph", idx);
+  h.nvis_real = nvis;
+  h.ncph = ncph;
+  h.cpext = cpext;
+  h.nvis_k = nvis + (cpext ? 3 * ncph : 0);
+  h.vis.assign((size_t)h.nvis_k * 4, 0.0f);
+  for (int i = 0; i < nvis; ++i) {
+    h.vis[4 * i + 0] = s.coor[2 * i];
+    h.vis[4 * i + 1] = s.coor[2 * i + 1];
+    h.vis[4 * i + 2] = s.dat[i];
+    h.vis[4 * i + 3] = s.std2r[i];
+  }
+  std::vector<int> tri((size_t)ncph * 3);
+  for (int j = 0; j < ncph; ++j) {
+    if (cpext) {
+      const float* c = s.coor + 2 * nvis + 4 * j;
+      const float uu[3] = {c[0], c[2], -(c[0] + c[2])};
+      const float vv[3] = {c[1], c[3], -(c[1] + c[3])};
+      for (int m = 0; m < 3; ++m) {
+        const int v = nvis + 3 * j + m;
+        h.vis[4 * v + 0] = uu[m];
+        h.vis[4 * v + 1] = vv[m];
+        tri[3 * j + m] = v;
+      }
+    } else {
+      for (int m = 0; m < 3; ++m) {
+        const int v = s.icoor[3 * j + m];
+        if (v < 0 || v >= nvis) FAIL(MHRE_EINVAL, "fit %d: icoor[%d] = %d outside [0, nvis)", idx, 3 * j + m, v);
+        tri[3 * j + m] = v;
+      }
+    }
+  }
+  h.ntile = (h.nvis_k + TILE - 1) / TILE;
+  // closures are handled after the tile holding their largest index, as long
+  // as all three indices are still in the RING-deep LDS window
+  std::vector<int> tile_of(ncph);
+  bool ok = true;
+  for (int j = 0; j < ncph; ++j) {
+    const int a = tri[3 * j], b = tri[3 * j + 1], c = tri[3 * j + 2];
+    const int mx = std::max(a, std::max(b, c)), mn = std::min(a, std::min(b, c));
+    tile_of[j] = mx / TILE;
+    if (mn < (tile_of[j] + 1) * TILE - RING) ok = false;
+  }
+  h.ring_ok = ok ? 1 : 0;
+  std::vector<int> order(ncph);
+  for (int j = 0; j < ncph; ++j) order[j] = j;
+  if (ok) std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return tile_of[x] < tile_of[y]; });
+  h.clo.assign((size_t)ncph * 4, 0);
+  h.clow.assign((size_t)ncph * 4, 0.0f);
+  for (int q = 0; q < ncph; ++q) {
+    const int j = order[q];
+    h.clo[4 * q + 0] = tri[3 * j];
+    h.clo[4 * q + 1] = tri[3 * j + 1];
+    h.clo[4 * q + 2] = tri[3 * j + 2];
+    h.clo[4 * q + 3] = nvis + j;   // datum index
+    const double obs = s.dat[nvis + j];
+    h.clow[4 * q + 0] = (float)cos(obs);
+    h.clow[4 * q + 1] = (float)sin(obs);
+    h.clow[4 * q + 2] = s.std2r[nvis + j];
+  }
+  h.clo_range.assign((size_t)h.ntile + 1, 0);
+  if (ok) {
+    int q = 0;
+    for (int t = 0; t <= h.ntile; ++t) {
+      while (q < ncph && tile_of[order[q]] < t) ++q;
+      h.clo_range[t] = q;
+    }
+  }
+  return MHRE_OK;
+}
+
+// MODEL_QUADSYS: coor = [Q (neq*n*n), b (neq*n)], dat = rhs, std2r = weights.
+int pack_quad(FitHost& h, int idx) {
+  const MhreFitSpec& s = h.s;
+  const int n = s.nptot, neq = s.ndat;
+  if (n != 2 && n != 4 && n != 8) FAIL(MHRE_EINVAL, "fit %d: quadratic systems support nptot in {2,4,8}", idx);
+  if (!s.coor || s.ncoor < neq * n * n + neq * n) FAIL(MHRE_EINVAL, "fit %d: coor too short for Q and b", idx);
+  const int ns = n * (n + 1) / 2, qs = ns + n + 2;
+  h.quad_n = n;
+  h.qcoef.assign((size_t)neq * qs, 0.0f);
+  for (int e = 0; e < neq; ++e) {
+    const float* Q = s.coor + (size_t)e * n * n;
+    const float* b = s.coor + (size_t)neq * n * n + (size_t)e * n;
+    float* c = &h.qcoef[(size_t)e * qs];
+    int m = 0;
+    for (int i = 0; i < n; ++i)
+      for (int j = i; j < n; ++j) c[m++] = (i == j) ? Q[i * n + i] : Q[i * n + j] + Q[j * n + i];
+    for (int i = 0; i < n; ++i) c[ns + i] = b[i];
+    c[ns + n] = s.dat[e];
+    c[ns + n + 1] = s.std2r[e];
+  }
+  return MHRE_OK;
+}
+
+// ------------------------------------------------------------- session
+
+struct KindWork {
+  int kind;
+  int quad_n;
+  std::vector<WorkItem> step, eval;
+  WorkItem* d_step = nullptr;
+  WorkItem* d_eval = nullptr;
+  int qslot = 0;
+};
+
+struct Session {
+  hipStream_t stream = nullptr;
+  std::vector<FitHost> fits;
+  char* arena = nullptr;
+  size_t arena_bytes = 0;
+  FitDesc* d_fits = nullptr;
+  RunClock* d_clk = nullptr;
+  unsigned long long* d_nevals = nullptr;   // [nfits][2]
+  int* d_err = nullptr;
+  WorkItem* d_init = nullptr;
+  int n_init = 0;
+  WorkItem* d_x = nullptr;
+  int n_x = 0;
+  std::vector<KindWork> kinds;
+  MhreOptions opt{};
+  int wpb = 4, persistent = 0, grid = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int graph_iters = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+  int iters_done = 0;
+  int* d_tr_dec = nullptr;
+  float* d_tr_chi2 = nullptr;
+  int* d_tr_ex = nullptr;
+  int tr_nit = 0, tr_it0 = 0;
+  std::vector<void*> extra;   // other device allocations
+};
+
+template <class T>
+T* at(Session* S, size_t off) { return reinterpret_cast<T*>(S->arena + off); }
+
+int resolve_kind(const MhreFitSpec& s, bool force_generic) {
+  switch (s.model) {
+    case MHRE_MODEL_SGRA9: return K_SGRA9;
+    case MHRE_MODEL_SGRA13: return K_SGRA13;
+    case MHRE_MODEL_USER: return K_USER;
+    case MHRE_MODEL_QUADSYS: return K_QUAD;
+    default: break;
+  }
+  if (kNativeEht && !force_generic && !env_flag("MHRE_FORCE_GENERIC")) {
+    if (s.imodel == 1) return K_SGRA9;
+    if (s.imodel == 2) return K_SGRA13;
+  }
+  return K_USER;
+}
+
+void destroy_session(Session* S) {
+  if (!S) return;
+  if (S->stream) hipStreamSynchronize(S->stream);
+  if (S->gexec) hipGraphExecDestroy(S->gexec);
+  if (S->graph) hipGraphDestroy(S->graph);
+  // every device allocation except the arena is owned by S->extra
+  for (void* p : S->extra) hipFree(p);
+  if (S->arena) hipFree(S->arena);
+  if (S->ev0) hipEventDestroy(S->ev0);
+  if (S->ev1) hipEventDestroy(S->ev1);
+  if (S->stream) hipStreamDestroy(S->stream);
+  delete S;
+}
+
+template <class T>
+int dev_alloc(Session* S, T** p, size_t n) {
+  void* q = nullptr;
+  HIPCHK(hipMalloc(&q, std::max<size_t>(n * sizeof(T), 16)));
+  S->extra.push_back(q);
+  *p = reinterpret_cast<T*>(q);
+  return MHRE_OK;
+}
+
+template <class T>
+int h2d(Session* S, size_t off, const T* src, size_t n) {
+  if (!src || n == 0) return MHRE_OK;
+  HIPCHK(hipMemcpyAsync(S->arena + off, src, n * sizeof(T), hipMemcpyHostToDevice, S->stream));
+  return MHRE_OK;
+}
+
+template <class T>
+int d2h(Session* S, T* dst, size_t off, size_t n) {
+  if (!dst || n == 0) return MHRE_OK;
+  HIPCHK(hipMemcpyAsync(dst, S->arena + off, n * sizeof(T), hipMemcpyDeviceToHost, S->stream));
+  return MHRE_OK;
+}
+
+#define CHK(x)              \
+  do {                      \
+    int r_ = (x);           \
+    if (r_ != MHRE_OK) return r_; \
+  } while (0)
+
+StepLaunch make_launch(Session* S, const KindWork& k) {
+  StepLaunch L{};
+  L.fits = S->d_fits;
+  L.work = k.d_step;
+  L.nwork = (int)k.step.size();
+  L.clk = S->d_clk;
+  L.wpb = S->wpb;
+  L.persistent = S->persistent;
+  L.grid = S->grid;
+  L.qslot = k.qslot;
+  L.tr_dec = S->d_tr_dec;
+  L.tr_chi2 = S->d_tr_chi2;
+  L.tr_ex = S->d_tr_ex;
+  L.tr_it0 = S->tr_it0;
+  L.tr_nit = S->tr_nit;
+  return L;
+}
+
+int launch_iteration(Session* S) {
+  for (auto& k : S->kinds) {
+    const StepLaunch L = make_launch(S, k);
+    hipError_t e = hipSuccess;
+    if (k.kind == K_SGRA9) e = launch_step_eht(MODEL_SGRA9, L, S->stream);
+    else if (k.kind == K_SGRA13) e = launch_step_eht(MODEL_SGRA13, L, S->stream);
+    else if (k.kind == K_QUAD) e = launch_step_quad(k.quad_n, L, S->stream);
+    else e = launch_step_user(L, S->stream);
+    HIPCHK(e);
+  }
+  StepLaunch L{};
+  L.fits = S->d_fits;
+  L.clk = S->d_clk;
+  L.tr_ex = S->d_tr_ex;
+  L.tr_it0 = S->tr_it0;
+  L.tr_nit = S->tr_nit;
+  HIPCHK(launch_exchange(L, S->d_x, S->n_x, S->stream));
+  return MHRE_OK;
+}
+
+int launch_eval_all(Session* S, int outputs, int respect_flag) {
+  for (auto& k : S->kinds) {
+    const int n = (int)k.eval.size();
+    hipError_t e = hipSuccess;
+    if (k.kind == K_SGRA9) e = launch_eval_eht(MODEL_SGRA9, S->d_fits, k.d_eval, n, outputs, respect_flag, S->stream);
+    else if (k.kind == K_SGRA13) e = launch_eval_eht(MODEL_SGRA13, S->d_fits, k.d_eval, n, outputs, respect_flag, S->stream);
+    else if (k.kind == K_QUAD) e = launch_eval_quad(k.quad_n, S->d_fits, k.d_eval, n, S->stream);
+    else e = launch_eval_user(S->d_fits, k.d_eval, n, 0, respect_flag, S->stream);
+    HIPCHK(e);
+  }
+  return MHRE_OK;
+}
+
+int check_model_error(Session* S) {
+  int err = 0;
+  HIPCHK(hipMemcpyAsync(&err, S->d_err, sizeof(int), hipMemcpyDeviceToHost, S->stream));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  if (err) FAIL(MHRE_EMODEL, "model() returned error code %d (e.g. an unknown imodel)", err);
+  return MHRE_OK;
+}
+
+int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session** out, int eval_only) {
+  *out = nullptr;
+  (void)hipGetLastError();   // a stale error of other code must not be reported as ours
+  if (!specs || nfits <= 0) FAIL(MHRE_EINVAL, "no fits given");
+  Session* S = new Session();
+  struct Guard {
+    Session*& s;
+    ~Guard() { if (s) destroy_session(s); }
+  } guard{S};
+  if (opt) S->opt = *opt;
+  S->wpb = S->opt.waves_per_block == 1 || S->opt.waves_per_block == 2 ? S->opt.waves_per_block : 4;
+  S->persistent = S->opt.persistent > 0 ? 1 : 0;
+  HIPCHK(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
+  HIPCHK(hipEventCreate(&S->ev0));
+  HIPCHK(hipEventCreate(&S->ev1));
+
+  const int it0 = specs[0].it0;
+  S->fits.resize(nfits);
+  Arena A;
+  for (int i = 0; i < nfits; ++i) {
+    FitHost& h = S->fits[i];
+    h.s = specs[i];
+    if (!eval_only) CHK(validate(h.s, i));
+    if (h.s.it0 != it0) FAIL(MHRE_EINVAL, "fit %d: all fits of a session must share it0", i);
+    h.kind = resolve_kind(h.s, S->opt.force_generic != 0);
+    h.nchain = h.s.nbeta * h.s.nseq;
+    if (h.kind == K_SGRA9 || h.kind == K_SGRA13) CHK(pack_eht(h, i));
+    if (h.kind == K_QUAD) CHK(pack_quad(h, i));
+    if (h.kind == K_USER && h.s.npass <= 0) FAIL(MHRE_EINVAL, "fit %d: npass must be positive", i);
+    const MhreFitSpec& s = h.s;
+    const size_t C = (size_t)h.nchain, P = (size_t)s.nprm, N = (size_t)s.nptot;
+    h.o_vis = A.reserve<float>(h.vis.size());
+    h.o_clo = A.reserve<int>(h.clo.size());
+    h.o_clow = A.reserve<float>(h.clow.size());
+    h.o_crange = A.reserve<int>(h.clo_range.size());
+    h.o_qcoef = A.reserve<float>(h.qcoef.size());
+    h.o_coor = A.reserve<float>(s.coor ? s.ncoor : 0);
+    h.o_dat = A.reserve<float>(s.ndat);
+    h.o_std2r = A.reserve<float>(s.ndat);
+    h.o_icoor = A.reserve<int>(s.icoor ? s.nicoor : 0);
+    h.o_idat = A.reserve<int>(s.idat ? s.nidat : 0);
+    h.o_pdescr = A.reserve<int>(N);
+    h.o_ivar = A.reserve<int>(P);
+    h.o_invar = A.reserve<int>(N);
+    h.o_pmint = A.reserve<float>(N);
+    h.o_pmaxt = A.reserve<float>(N);
+    h.o_beta = A.reserve<float>(s.nbeta);
+    h.o_pcur = A.reserve<float>(C * N);
+    h.o_chi2c = A.reserve<float>(C);
+    h.o_pstp = A.reserve<float>(P * C);
+    h.o_ncnt = A.reserve<int>(P * C);
+    h.o_nacpt = A.reserve<int>(P * C);
+    h.o_nhist = A.reserve<int>((size_t)s.nadj * P * C);
+    h.o_nexch = A.reserve<int>(C);
+    h.o_sumh = A.reserve<int>(P * C);
+    h.o_tcur = A.reserve<int>(C);
+    h.o_flag = A.reserve<int>(C);
+    h.o_ptent = A.reserve<float>(C);
+    h.o_ptentn = A.reserve<int>(C);
+    h.o_pout = A.reserve<float>(P * C * (size_t)s.niter);
+    h.o_tout = A.reserve<int>(C * (size_t)s.niter);
+    h.o_chi2 = A.reserve<float>((size_t)s.nseq * s.niter);
+    h.has_datm = h.kind == K_USER || s.datm != nullptr;
+    const size_t ndatm = h.has_datm ? std::max<size_t>((size_t)std::max(s.ndatm, 0), C * s.ndat) : 0;
+    h.o_datm = A.reserve<float>(ndatm);
+    h.o_chi2m = A.reserve<float>(h.has_datm ? C * s.ndat : 0);
+    h.o_mc = A.reserve<CCalcModel>(1);
+    h.d.ndatm = (long long)ndatm;
+  }
+  S->arena_bytes = A.bytes;
+  HIPCHK(hipMalloc((void**)&S->arena, A.bytes));
+  CHK(dev_alloc(S, &S->d_fits, nfits));
+  CHK(dev_alloc(S, &S->d_clk, 1));
+  CHK(dev_alloc(S, &S->d_nevals, 2 * (size_t)nfits));
+  CHK(dev_alloc(S, &S->d_err, 1));
+  HIPCHK(hipMemsetAsync(S->d_nevals, 0, 2 * sizeof(unsigned long long) * nfits, S->stream));
+  HIPCHK(hipMemsetAsync(S->d_err, 0, sizeof(int), S->stream));
+  RunClock clk{};
+  clk.it = it0;
+  HIPCHK(hipMemcpyAsync(S->d_clk, &clk, sizeof(clk), hipMemcpyHostToDevice, S->stream));
+
+  std::vector<FitDesc> descs(nfits);
+  for (int i = 0; i < nfits; ++i) {
+    FitHost& h = S->fits[i];
+    const MhreFitSpec& s = h.s;
+    const size_t C = (size_t)h.nchain, P = (size_t)s.nprm, N = (size_t)s.nptot;
+    FitDesc& d = h.d;
+    const long long ndatm = d.ndatm;
+    d = FitDesc{};
+    d.ndatm = ndatm;
+    d.model = h.kind == K_SGRA9 ? MODEL_SGRA9 : h.kind == K_SGRA13 ? MODEL_SGRA13 : h.kind == K_QUAD ? MODEL_QUADSYS : MODEL_USER;
+    d.imodel = s.imodel;
+    d.nbeta = s.nbeta; d.nseq = s.nseq; d.nptot = s.nptot; d.nprm = s.nprm; d.nadj = s.nadj; d.npass = s.npass;
+    d.ndat = s.ndat;
+    d.nvis = h.nvis_k; d.ncph = h.ncph; d.ntile = h.ntile; d.ring_ok = h.ring_ok;
+    d.nburn = s.nburn; d.niter = s.niter; d.nchain = h.nchain;
+    d.key0 = (uint32_t)(s.seed & 0xffffffffull);
+    d.key1 = (uint32_t)(s.seed >> 32);
+    d.adapt_target = s.adapt_target > 0.0f ? s.adapt_target : 0.30f;
+    d.quad_n = h.quad_n; d.quad_neq = s.ndat;
+    d.vis = at<float4>(S, h.o_vis); d.clo = at<int4>(S, h.o_clo); d.clow = at<float4>(S, h.o_clow);
+    d.clo_range = at<int>(S, h.o_crange); d.qcoef = at<float>(S, h.o_qcoef);
+    d.coor = at<float>(S, h.o_coor); d.dat = at<float>(S, h.o_dat); d.std2r = at<float>(S, h.o_std2r);
+    d.icoor = at<int>(S, h.o_icoor); d.idat = at<int>(S, h.o_idat);
+    d.pdescr = at<int>(S, h.o_pdescr); d.ivar = at<int>(S, h.o_ivar); d.invar = at<int>(S, h.o_invar);
+    d.pmint = at<float>(S, h.o_pmint); d.pmaxt = at<float>(S, h.o_pmaxt); d.beta = at<float>(S, h.o_beta);
+    d.pcur = at<float>(S, h.o_pcur); d.chi2c = at<float>(S, h.o_chi2c); d.pstp = at<float>(S, h.o_pstp);
+    d.n_cnt = at<int>(S, h.o_ncnt); d.n_acpt = at<int>(S, h.o_nacpt); d.n_hist = at<int>(S, h.o_nhist);
+    d.n_exch = at<int>(S, h.o_nexch); d.sumh = at<int>(S, h.o_sumh); d.tcur = at<int>(S, h.o_tcur);
+    d.flag = at<int>(S, h.o_flag); d.ptent = at<float>(S, h.o_ptent); d.ptentn = at<int>(S, h.o_ptentn);
+    d.pout = at<float>(S, h.o_pout); d.tout = at<int>(S, h.o_tout); d.chi2 = at<float>(S, h.o_chi2);
+    d.datm = h.has_datm ? at<float>(S, h.o_datm) : nullptr;
+    d.chi2m = h.has_datm ? at<float>(S, h.o_chi2m) : nullptr;
+    d.vscratch = nullptr;
+    d.mc = at<CCalcModel>(S, h.o_mc);
+    d.nevals = S->d_nevals + 2 * i;
+    d.errflag = S->d_err;
+    descs[i] = d;
+
+    CHK(h2d(S, h.o_vis, h.vis.data(), h.vis.size()));
+    CHK(h2d(S, h.o_clo, h.clo.data(), h.clo.size()));
+    CHK(h2d(S, h.o_clow, h.clow.data(), h.clow.size()));
+    CHK(h2d(S, h.o_crange, h.clo_range.data(), h.clo_range.size()));
+    CHK(h2d(S, h.o_qcoef, h.qcoef.data(), h.qcoef.size()));
+    CHK(h2d(S, h.o_coor, s.coor, s.coor ? s.ncoor : 0));
+    CHK(h2d(S, h.o_dat, s.dat, s.ndat));
+    CHK(h2d(S, h.o_std2r, s.std2r, s.ndat));
+    CHK(h2d(S, h.o_icoor, s.icoor, s.icoor ? s.nicoor : 0));
+    CHK(h2d(S, h.o_idat, s.idat, s.idat ? s.nidat : 0));
+    CHK(h2d(S, h.o_pdescr, s.pdescr, N));
+    CHK(h2d(S, h.o_ivar, s.ivar, P));
+    CHK(h2d(S, h.o_invar, s.invar, N));
+    CHK(h2d(S, h.o_pmint, s.pmint, N));
+    CHK(h2d(S, h.o_pmaxt, s.pmaxt, N));
+    CHK(h2d(S, h.o_beta, s.beta, s.nbeta));
+    CHK(h2d(S, h.o_pcur, s.pcur, C * N));
+    CHK(h2d(S, h.o_chi2c, s.chi2c, C));
+    CHK(h2d(S, h.o_pstp, s.pstp, P * C));
+    CHK(h2d(S, h.o_ncnt, s.n_cnt, P * C));
+    CHK(h2d(S, h.o_nacpt, s.n_acpt, P * C));
+    CHK(h2d(S, h.o_nhist, s.n_hist, (size_t)s.nadj * P * C));
+    CHK(h2d(S, h.o_nexch, s.n_exch, C));
+    CHK(h2d(S, h.o_tcur, s.tcur, C));
+    CHK(h2d(S, h.o_flag, s.flag, C));
+    if (s.ptent) CHK(h2d(S, h.o_ptent, s.ptent, C));
+    if (s.ptentn) CHK(h2d(S, h.o_ptentn, s.ptentn, C));
+    if (h.has_datm && s.datm) CHK(h2d(S, h.o_datm, s.datm, (size_t)std::min<long long>(d.ndatm, s.ndatm)));
+    if (h.has_datm && s.chi2m) CHK(h2d(S, h.o_chi2m, s.chi2m, C * s.ndat));
+    // running accept sums over the nadj-deep history (kept exactly equal to
+    // sum_a n_hist[a][k][chain])
+    h.sumh.assign(P * C, 0);
+    for (int a = 0; a < s.nadj; ++a)
+      for (size_t q = 0; q < P * C; ++q) h.sumh[q] += s.n_hist[(size_t)a * P * C + q];
+    CHK(h2d(S, h.o_sumh, h.sumh.data(), h.sumh.size()));
+    CCalcModel mc{};
+    mc.imodel = s.imodel; mc.ncoor = s.ncoor; mc.ndat = s.ndat; mc.nicoor = s.nicoor; mc.nidat = s.nidat;
+    mc.ndatm = (int)std::min<long long>(d.ndatm, 0x7fffffff); mc.nptot = s.nptot; mc.nbeta = s.nbeta; mc.nseq = s.nseq;
+    mc.nchi2m = (int)(C * s.ndat);
+    mc.coor = const_cast<float*>(d.coor); mc.dat = const_cast<float*>(d.dat); mc.std2r = const_cast<float*>(d.std2r);
+    mc.icoor = const_cast<int*>(d.icoor); mc.idat = const_cast<int*>(d.idat);
+    mc.datm = d.datm; mc.chi2m = d.chi2m; mc.pcur = d.pcur; mc.flag = d.flag; mc.chi2c = d.chi2c;
+    HIPCHK(hipMemcpyAsync(S->arena + h.o_mc, &mc, sizeof(mc), hipMemcpyHostToDevice, S->stream));
+    HIPCHK(hipStreamSynchronize(S->stream));   // host staging vectors must outlive the copies
+  }
+  HIPCHK(hipMemcpyAsync(S->d_fits, descs.data(), sizeof(FitDesc) * nfits, hipMemcpyHostToDevice, S->stream));
+
+  // work lists: fits grouped per kind, heaviest fits first
+  std::vector<WorkItem> init, xw;
+  const int kinds_order[4] = {K_SGRA9, K_SGRA13, K_QUAD, K_USER};
+  int qslot = 0;
+  for (int kk : kinds_order) {
+    std::vector<int> idx;
+    for (int i = 0; i < nfits; ++i)
+      if (S->fits[i].kind == kk) idx.push_back(i);
+    if (idx.empty()) continue;
+    std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
+      const FitHost &x = S->fits[a], &y = S->fits[b];
+      return (long long)x.nvis_k + x.s.ndat > (long long)y.nvis_k + y.s.ndat;
+    });
+    if (kk == K_QUAD) {
+      // one kernel per compiled system size
+      for (int n : {2, 4, 8}) {
+        KindWork kw;
+        kw.kind = kk;
+        kw.quad_n = n;
+        for (int i : idx) {
+          if (S->fits[i].quad_n != n) continue;
+          const int C = S->fits[i].nchain;
+          for (int c = 0; c < C; ++c) { kw.step.push_back({i, c}); kw.eval.push_back({i, c}); }
+          while (kw.step.size() % 64) kw.step.push_back({i, -1});
+        }
+        if (!kw.eval.empty()) S->kinds.push_back(std::move(kw));
+      }
+      continue;
+    }
+    KindWork kw;
+    kw.kind = kk;
+    kw.qslot = qslot++;
+    for (int i : idx)
+      for (int c = 0; c < S->fits[i].nchain; ++c) { kw.step.push_back({i, c}); kw.eval.push_back({i, c}); }
+    S->kinds.push_back(std::move(kw));
+  }
+  for (int i = 0; i < nfits; ++i) {
+    const FitHost& h = S->fits[i];
+    if (h.s.init_chains)
+      for (int c = 0; c < h.nchain; ++c) init.push_back({i, c});
+    for (int q = 0; q < h.s.nseq; ++q) xw.push_back({i, q});
+  }
+  for (auto& k : S->kinds) {
+    CHK(dev_alloc(S, &k.d_step, k.step.size()));
+    CHK(dev_alloc(S, &k.d_eval, k.eval.size()));
+    HIPCHK(hipMemcpyAsync(k.d_step, k.step.data(), k.step.size() * sizeof(WorkItem), hipMemcpyHostToDevice, S->stream));
+    HIPCHK(hipMemcpyAsync(k.d_eval, k.eval.data(), k.eval.size() * sizeof(WorkItem), hipMemcpyHostToDevice, S->stream));
+  }
+  S->n_init = (int)init.size();
+  S->n_x = (int)xw.size();
+  CHK(dev_alloc(S, &S->d_init, init.size()));
+  CHK(dev_alloc(S, &S->d_x, xw.size()));
+  if (!init.empty())
+    HIPCHK(hipMemcpyAsync(S->d_init, init.data(), init.size() * sizeof(WorkItem), hipMemcpyHostToDevice, S->stream));
+  HIPCHK(hipMemcpyAsync(S->d_x, xw.data(), xw.size() * sizeof(WorkItem), hipMemcpyHostToDevice, S->stream));
+
+  // persistent grid: a few work-groups per CU
+  int dev = 0, ncu = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  S->grid = std::max(1, ncu * (16 / S->wpb));
+
+  // decision trace (single fit)
+  if (S->opt.trace_iters > 0 && nfits == 1) {
+    const FitHost& h = S->fits[0];
+    S->tr_nit = S->opt.trace_iters;
+    S->tr_it0 = it0;
+    const size_t nd = (size_t)S->tr_nit * h.s.nprm * h.nchain;
+    CHK(dev_alloc(S, &S->d_tr_dec, nd));
+    CHK(dev_alloc(S, &S->d_tr_chi2, nd));
+    CHK(dev_alloc(S, &S->d_tr_ex, (size_t)S->tr_nit * h.nchain));
+    HIPCHK(hipMemsetAsync(S->d_tr_dec, 0xff, nd * sizeof(int), S->stream));
+    HIPCHK(hipMemsetAsync(S->d_tr_ex, 0xff, (size_t)S->tr_nit * h.nchain * sizeof(int), S->stream));
+  }
+
+  if (!eval_only) {
+    // init_pcur + initial chi^2 (src/gpu_mcmc.cu:714-719)
+    HIPCHK(launch_init(S->d_fits, S->d_init, S->n_init, S->stream));
+    if (S->n_init > 0) {
+      for (auto& k : S->kinds) {
+        std::vector<WorkItem> ev;
+        for (const WorkItem& w : k.eval)
+          if (S->fits[w.fit].s.init_chains) ev.push_back(w);
+        if (ev.empty()) continue;
+        WorkItem* dev_ev = nullptr;
+        CHK(dev_alloc(S, &dev_ev, ev.size()));
+        HIPCHK(hipMemcpyAsync(dev_ev, ev.data(), ev.size() * sizeof(WorkItem), hipMemcpyHostToDevice, S->stream));
+        const int n = (int)ev.size();
+        hipError_t e;
+        if (k.kind == K_SGRA9) e = launch_eval_eht(MODEL_SGRA9, S->d_fits, dev_ev, n, 0, 0, S->stream);
+        else if (k.kind == K_SGRA13) e = launch_eval_eht(MODEL_SGRA13, S->d_fits, dev_ev, n, 0, 0, S->stream);
+        else if (k.kind == K_QUAD) e = launch_eval_quad(k.quad_n, S->d_fits, dev_ev, n, S->stream);
+        else e = launch_eval_user(S->d_fits, dev_ev, n, 0, 0, S->stream);
+        HIPCHK(e);
+        HIPCHK(hipStreamSynchronize(S->stream));
+      }
+    }
+    HIPCHK(hipStreamSynchronize(S->stream));
+    CHK(check_model_error(S));
+  }
+  HIPCHK(hipStreamSynchronize(S->stream));
+  *out = S;
+  S = nullptr;   // released from the guard
+  return MHRE_OK;
+}
+
+int run(Session* S, int niter) {
+  if (niter <= 0) return MHRE_OK;
+  (void)hipGetLastError();
+  HIPCHK(hipEventRecord(S->ev0, S->stream));
+  int left = niter;
+  const int G = S->opt.graph_iters;
+  if (G > 0 && niter >= G) {
+    if (!S->gexec) {
+      HIPCHK(hipStreamBeginCapture(S->stream, hipStreamCaptureModeThreadLocal));
+      int rc = MHRE_OK;
+      for (int i = 0; i < G && rc == MHRE_OK; ++i) rc = launch_iteration(S);
+      hipGraph_t g = nullptr;
+      const hipError_t ec = hipStreamEndCapture(S->stream, &g);
+      if (rc != MHRE_OK) return rc;
+      HIPCHK(ec);
+      S->graph = g;
+      HIPCHK(hipGraphInstantiate(&S->gexec, g, nullptr, nullptr, 0));
+    }
+    while (left >= G) {
+      HIPCHK(hipGraphLaunch(S->gexec, S->stream));
+      left -= G;
+    }
+  }
+  for (int i = 0; i < left; ++i) CHK(launch_iteration(S));
+  HIPCHK(hipEventRecord(S->ev1, S->stream));
+  S->iters_done += niter;
+  return MHRE_OK;
+}
+
+int download(Session* S, int model_data) {
+  HIPCHK(hipStreamSynchronize(S->stream));
+  for (FitHost& h : S->fits) {
+    const MhreFitSpec& s = h.s;
+    const size_t C = (size_t)h.nchain, P = (size_t)s.nprm, N = (size_t)s.nptot;
+    CHK(d2h(S, s.pcur, h.o_pcur, C * N));
+    CHK(d2h(S, s.chi2c, h.o_chi2c, C));
+    CHK(d2h(S, s.pstp, h.o_pstp, P * C));
+    CHK(d2h(S, s.n_cnt, h.o_ncnt, P * C));
+    CHK(d2h(S, s.n_acpt, h.o_nacpt, P * C));
+    CHK(d2h(S, s.n_hist, h.o_nhist, (size_t)s.nadj * P * C));
+    CHK(d2h(S, s.n_exch, h.o_nexch, C));
+    CHK(d2h(S, s.tcur, h.o_tcur, C));
+    CHK(d2h(S, s.flag, h.o_flag, C));
+    CHK(d2h(S, s.ptent, h.o_ptent, C));
+    CHK(d2h(S, s.ptentn, h.o_ptentn, C));
+    CHK(d2h(S, s.pout, h.o_pout, P * C * (size_t)s.niter));
+    CHK(d2h(S, s.tout, h.o_tout, C * (size_t)s.niter));
+    CHK(d2h(S, s.chi2, h.o_chi2, (size_t)s.nseq * s.niter));
+  }
+  HIPCHK(hipStreamSynchronize(S->stream));
+  if (model_data) {
+    // model data and chi^2 terms of each chain's final state; chi2c is
+    // recomputed identically and already copied out above
+    std::vector<float> keep;
+    CHK(launch_eval_all(S, 1, 0));
+    HIPCHK(hipStreamSynchronize(S->stream));
+    for (FitHost& h : S->fits) {
+      const MhreFitSpec& s = h.s;
+      if (!h.has_datm) continue;
+      const size_t C = (size_t)h.nchain;
+      if (s.datm) CHK(d2h(S, s.datm, h.o_datm, (size_t)std::min<long long>(h.d.ndatm, s.ndatm)));
+      if (s.chi2m) CHK(d2h(S, s.chi2m, h.o_chi2m, C * s.ndat));
+    }
+  }
+  // resume records in rndst: seed (8 B), next iteration (4 B), chain (4 B)
+  int it = 0;
+  HIPCHK(hipMemcpyAsync(&it, &S->d_clk->it, sizeof(int), hipMemcpyDeviceToHost, S->stream));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  for (FitHost& h : S->fits) {
+    if (!h.s.rndst) continue;
+    for (int c = 0; c < h.nchain; ++c) {
+      unsigned char* r = h.s.rndst + (size_t)c * 48;
+      memset(r, 0, 48);
+      memcpy(r, &h.s.seed, 8);
+      memcpy(r + 8, &it, 4);
+      memcpy(r + 12, &c, 4);
+    }
+  }
+  if (S->d_tr_dec && S->opt.trace_dec) {
+    const FitHost& h = S->fits[0];
+    const size_t nd = (size_t)S->tr_nit * h.s.nprm * h.nchain;
+    HIPCHK(hipMemcpy(S->opt.trace_dec, S->d_tr_dec, nd * sizeof(int), hipMemcpyDeviceToHost));
+    if (S->opt.trace_chi2) HIPCHK(hipMemcpy(S->opt.trace_chi2, S->d_tr_chi2, nd * sizeof(float), hipMemcpyDeviceToHost));
+    if (S->opt.trace_ex)
+      HIPCHK(hipMemcpy(S->opt.trace_ex, S->d_tr_ex, (size_t)S->tr_nit * h.nchain * sizeof(int), hipMemcpyDeviceToHost));
+  }
+  return check_model_error(S);
+}
+
+int stats(Session* S, MhreStats* st) {
+  if (!st) FAIL(MHRE_EINVAL, "stats pointer is NULL");
+  HIPCHK(hipStreamSynchronize(S->stream));
+  memset(st, 0, sizeof(*st));
+  float ms = 0.0f;
+  if (S->iters_done > 0 && hipEventElapsedTime(&ms, S->ev0, S->ev1) == hipSuccess) st->run_ms = ms;
+  std::vector<unsigned long long> ev(2 * S->fits.size());
+  HIPCHK(hipMemcpy(ev.data(), S->d_nevals, ev.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < S->fits.size(); ++i) {
+    const FitHost& h = S->fits[i];
+    st->chi2_evals += ev[2 * i];
+    st->proposals += ev[2 * i + 1];
+    if (h.kind == K_SGRA9 || h.kind == K_SGRA13) {
+      st->vis_evals += (double)ev[2 * i] * h.nvis_k;
+      st->cph_evals += (double)ev[2 * i] * h.ncph;
+    }
+  }
+  st->iters_done = S->iters_done;
+  st->nfits = (int)S->fits.size();
+  return MHRE_OK;
+}
+
+MhreFitSpec legacy_spec(float* coor, float* dat, float* std2r, int* icoor, int* idat, float* datm, float* chi2m,
+                        int* pdescr, int* ivar, int* invar, float* ptotal, float* pmint, float* pmaxt, float* pout,
+                        int* tout, float* chi2, float* chi2c, int* tcur, int* flag, int* n_acpt, int* n_exch,
+                        int* n_cnt, int* n_hist, float* beta, float* pstp, float* ptent, int* ptentn, float* pcur,
+                        unsigned char* rndst, unsigned long long seed, int imodel, int ncoor, int ndat, int nicoor,
+                        int nidat, int ndatm, int nptot, int nprm, int nadj, int npass, int nbeta, int nseq,
+                        int nburn, int niter) {
+  MhreFitSpec s{};
+  s.model = MHRE_MODEL_AUTO;
+  s.imodel = imodel;
+  s.coor = coor; s.dat = dat; s.std2r = std2r; s.icoor = icoor; s.idat = idat; s.datm = datm; s.chi2m = chi2m;
+  s.pdescr = pdescr; s.ivar = ivar; s.invar = invar; s.ptotal = ptotal; s.pmint = pmint; s.pmaxt = pmaxt;
+  s.pout = pout; s.tout = tout; s.chi2 = chi2; s.chi2c = chi2c; s.tcur = tcur; s.flag = flag;
+  s.n_acpt = n_acpt; s.n_exch = n_exch; s.n_cnt = n_cnt; s.n_hist = n_hist;
+  s.beta = beta; s.pstp = pstp; s.ptent = ptent; s.ptentn = ptentn; s.pcur = pcur; s.rndst = rndst;
+  s.seed = seed;
+  s.ncoor = ncoor; s.ndat = ndat; s.nicoor = nicoor; s.nidat = nidat; s.ndatm = ndatm;
+  s.nptot = nptot; s.nprm = nprm; s.nadj = nadj; s.npass = npass; s.nbeta = nbeta; s.nseq = nseq;
+  s.nburn = nburn; s.niter = niter;
+  s.adapt_target = 0.30f;
+  s.init_chains = 1;
+  s.it0 = 0;
+  return s;
+}
+
+}  // namespace
+
+// ===================================================================== C ABI
+
+extern "C" {
+
+const char* mhre_last_error(void) { return g_err.c_str(); }
+int mhre_version(void) { return 10000; }
+const char* mhre_user_model(void) { return user_model_name(); }
+int mhre_native_eht(void) { return kNativeEht; }
+
+int mhre_session_create(const MhreFitSpec* fits, int nfits, const MhreOptions* opt, void** session) {
+  if (!session) FAIL(MHRE_EINVAL, "session pointer is NULL");
+  Session* S = nullptr;
+  const int rc = create(fits, nfits, opt, &S, 0);
+  *session = S;
+  return rc;
+}
+
+int mhre_session_run(void* session, int niter) {
+  if (!session) FAIL(MHRE_EINVAL, "NULL session");
+  return run(static_cast<Session*>(session), niter);
+}
+
+int mhre_session_sync(void* session) {
+  if (!session) FAIL(MHRE_EINVAL, "NULL session");
+  HIPCHK(hipStreamSynchronize(static_cast<Session*>(session)->stream));
+  return MHRE_OK;
+}
+
+int mhre_session_download(void* session, int model_data) {
+  if (!session) FAIL(MHRE_EINVAL, "NULL session");
+  return download(static_cast<Session*>(session), model_data);
+}
+
+int mhre_session_stats(void* session, MhreStats* st) {
+  if (!session) FAIL(MHRE_EINVAL, "NULL session");
+  return stats(static_cast<Session*>(session), st);
+}
+
+int mhre_session_destroy(void* session) {
+  destroy_session(static_cast<Session*>(session));
+  return MHRE_OK;
+}
+
+int mhre_run_batch(const MhreFitSpec* fits, int nfits, const MhreOptions* opt, MhreStats* st) {
+  Session* S = nullptr;
+  int rc = create(fits, nfits, opt, &S, 0);
+  if (rc != MHRE_OK) return rc;
+  int total = 0;
+  for (int i = 0; i < nfits; ++i) total = std::max(total, fits[i].nburn + fits[i].niter - fits[i].it0);
+  rc = run(S, total);
+  bool model_data = false;
+  for (int i = 0; i < nfits; ++i) model_data = model_data || fits[i].datm != nullptr;
+  if (rc == MHRE_OK) rc = download(S, model_data ? 1 : 0);
+  if (rc == MHRE_OK && st) rc = stats(S, st);
+  destroy_session(S);
+  return rc;
+}
+
+int mcmc_cuda(float* coor, float* dat, float* std2r, int* icoor, int* idat, float* datm, float* chi2m,
+              int* pdescr, int* ivar, int* invar, float* ptotal, float* pmint, float* pmaxt, float* pout,
+              int* tout, float* chi2, float* chi2c, int* tcur, int* flag, int* n_acpt, int* n_exch, int* n_cnt,
+              int* n_hist, float* beta, float* pstp, float* ptent, int* ptentn, float* pcur, unsigned char* rndst,
+              unsigned long long seed, int imodel, int ncoor, int ndat, int nicoor, int nidat, int ndatm,
+              int nptot, int nprm, int nadj, int npass, int nbeta, int nseq, int nburn, int niter) {
+  const MhreFitSpec s = legacy_spec(coor, dat, std2r, icoor, idat, datm, chi2m, pdescr, ivar, invar, ptotal, pmint,
+                                    pmaxt, pout, tout, chi2, chi2c, tcur, flag, n_acpt, n_exch, n_cnt, n_hist, beta,
+                                    pstp, ptent, ptentn, pcur, rndst, seed, imodel, ncoor, ndat, nicoor, nidat, ndatm,
+                                    nptot, nprm, nadj, npass, nbeta, nseq, nburn, niter);
+  MhreOptions opt{};
+  opt.persistent = -1;
+  return mhre_run_batch(&s, 1, &opt, nullptr);
+}
+
+int calc_modchi2(float* coor, float* dat, float* std2r, int* icoor, int* idat, float* datm, float* chi2m,
+                 float* pcur, float* chi2c, int* flag, int imodel, int ncoor, int ndat, int nicoor, int nidat,
+                 int ndatm, int nptot, int nbeta, int nseq) {
+  if (!pcur || !chi2c || !flag || !dat || !std2r || nbeta <= 0 || nseq <= 0 || ndat <= 0 || nptot <= 0)
+    FAIL(MHRE_EINVAL, "calc_modchi2: invalid arguments");
+  if (nptot > MAXP) FAIL(MHRE_EINVAL, "calc_modchi2: nptot %d exceeds %d", nptot, MAXP);
+  const int C = nbeta * nseq;
+  if (ndatm < C * ndat) FAIL(MHRE_EINVAL, "calc_modchi2: ndatm %d < nbeta*nseq*ndat", ndatm);
+  std::vector<int> pdescr(nptot, 0), invar(nptot, -1), ivar(1, 0);
+  std::vector<float> pmm(nptot, 0.0f), beta(nbeta, 1.0f), zf(C, 0.0f);
+  std::vector<int> zi(C, 0);
+  MhreFitSpec s{};
+  s.model = MHRE_MODEL_AUTO; s.imodel = imodel;
+  s.coor = coor; s.dat = dat; s.std2r = std2r; s.icoor = icoor; s.idat = idat; s.datm = datm; s.chi2m = chi2m;
+  s.pdescr = pdescr.data(); s.ivar = ivar.data(); s.invar = invar.data(); s.pmint = pmm.data(); s.pmaxt = pmm.data();
+  s.beta = beta.data(); s.pcur = pcur; s.chi2c = chi2c; s.flag = flag;
+  s.pstp = zf.data(); s.n_cnt = zi.data(); s.n_acpt = zi.data(); s.n_hist = zi.data(); s.n_exch = zi.data();
+  s.tcur = zi.data(); s.ptent = zf.data(); s.ptentn = zi.data();
+  s.ncoor = ncoor; s.ndat = ndat; s.nicoor = nicoor; s.nidat = nidat; s.ndatm = ndatm;
+  s.nptot = nptot; s.nprm = 1; s.nadj = 1; s.npass = 2; s.nbeta = nbeta; s.nseq = nseq;
+  Session* S = nullptr;
+  int rc = create(&s, 1, nullptr, &S, 1);
+  if (rc != MHRE_OK) return rc;
+  // the reference runs passes 0 and 1 (gpu_mcmc.cu:778-779)
+  rc = launch_eval_all(S, 1, 1);
+  const FitHost& h = S->fits[0];
+  if (rc == MHRE_OK) rc = d2h(S, chi2c, h.o_chi2c, C);
+  if (rc == MHRE_OK && datm) rc = d2h(S, datm, h.o_datm, (size_t)std::min<long long>(h.d.ndatm, ndatm));
+  if (rc == MHRE_OK && chi2m) rc = d2h(S, chi2m, h.o_chi2m, (size_t)C * ndat);
+  if (rc == MHRE_OK) rc = check_model_error(S);
+  destroy_session(S);
+  return rc;
+}
+
+int calc_model(float* coor, int* idat, float* datm, float* pcur, int imodel, int ncoor, int ndat, int nidat,
+               int ndatm, int nptot, int nbeta, int nseq) {
+  if (!pcur || !datm || !coor || nbeta <= 0 || nseq <= 0 || ndat <= 0 || nptot <= 0)
+    FAIL(MHRE_EINVAL, "calc_model: invalid arguments");
+  if (nptot > MAXP) FAIL(MHRE_EINVAL, "calc_model: nptot %d exceeds %d", nptot, MAXP);
+  const int C = nbeta * nseq;
+  if (ndatm < C * ndat) FAIL(MHRE_EINVAL, "calc_model: ndatm %d < nbeta*nseq*ndat", ndatm);
+  std::vector<int> pdescr(nptot, 0), invar(nptot, -1), ivar(1, 0), flag(C, 1), zi(C, 0);
+  std::vector<float> pmm(nptot, 0.0f), beta(nbeta, 1.0f), zf(C, 0.0f), dat(ndat, 0.0f), w(ndat, 0.0f);
+  std::vector<int> idat_local(3, 0);
+  MhreFitSpec s{};
+  s.model = MHRE_MODEL_AUTO; s.imodel = imodel;
+  s.coor = coor; s.dat = dat.data(); s.std2r = w.data();
+  s.datm = datm; s.chi2m = nullptr;
+  idat_local[0] = ndat;
+  if (idat && nidat >= 1) idat_local[0] = idat[0];
+  s.idat = idat ? idat : idat_local.data(); s.nidat = idat ? nidat : 3;
+  s.pdescr = pdescr.data(); s.ivar = ivar.data(); s.invar = invar.data(); s.pmint = pmm.data(); s.pmaxt = pmm.data();
+  s.beta = beta.data(); s.pcur = pcur; s.chi2c = zf.data(); s.flag = flag.data();
+  s.pstp = zf.data(); s.n_cnt = zi.data(); s.n_acpt = zi.data(); s.n_hist = zi.data(); s.n_exch = zi.data();
+  s.tcur = zi.data(); s.ptent = zf.data(); s.ptentn = zi.data();
+  s.ncoor = ncoor; s.ndat = ndat; s.nicoor = 0; s.ndatm = ndatm;
+  s.nptot = nptot; s.nprm = 1; s.nadj = 1; s.npass = 1; s.nbeta = nbeta; s.nseq = nseq;
+  // model data only: closures are not part of calc_model (pass 0)
+  std::vector<int> idat_model(s.idat, s.idat + std::max(s.nidat, 1));
+  idat_model[0] = ndat;
+  if (idat_model.size() >= 3) idat_model[2] = 0;
+  s.idat = idat_model.data();
+  Session* S = nullptr;
+  int rc = create(&s, 1, nullptr, &S, 1);
+  if (rc != MHRE_OK) return rc;
+  const FitHost& h = S->fits[0];
+  hipError_t e = hipSuccess;
+  const KindWork& k = S->kinds[0];
+  if (h.kind == K_SGRA9 || h.kind == K_SGRA13)
+    e = launch_eval_eht(h.kind == K_SGRA9 ? MODEL_SGRA9 : MODEL_SGRA13, S->d_fits, k.d_eval, C, 1, 0, S->stream);
+  else
+    e = launch_eval_user(S->d_fits, k.d_eval, C, 1, 0, S->stream);
+  if (e != hipSuccess) {
+    set_err("calc_model launch: %s", hipGetErrorString(e));
+    rc = MHRE_EHIP;
+  }
+  if (rc == MHRE_OK) rc = d2h(S, datm, h.o_datm, (size_t)std::min<long long>(h.d.ndatm, ndatm));
+  if (rc == MHRE_OK) rc = check_model_error(S);
+  destroy_session(S);
+  return rc;
+}
+
+int reset_gpu(void) {
+  HIPCHK(hipDeviceReset());
+  return MHRE_OK;
+}
+
+int list_gpus(char* buf, int buflen) {
+  if (!buf || buflen <= 0) FAIL(MHRE_EINVAL, "list_gpus: empty buffer");
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n));
+  std::string out;
+  for (int i = 0; i < n; ++i) {
+    hipDeviceProp_t p;
+    HIPCHK(hipGetDeviceProperties(&p, i));
+    char line[512];
+    snprintf(line, sizeof(line), "%d: %s (%s), %d compute units, %.1f GiB, %d MHz\n", i, p.name, p.gcnArchName,
+             p.multiProcessorCount, p.totalGlobalMem / 1073741824.0, p.clockRate / 1000);
+    out += line;
+  }
+  snprintf(buf, (size_t)buflen, "%s", out.c_str());
+  return n;
+}
+
+}  // extern "C"

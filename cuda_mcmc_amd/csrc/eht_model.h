@@ -1,0 +1,198 @@
+// Native fp32 ring+Gaussian visibility models for the fused chi^2 kernels.
+//
+// Sgra9 restates examples/sgra_model_fit/model.cuh:322-462 (sgra_model9) and
+// Sgra13 restates model.cuh:120-315 (sgra_model13).  Per-chain invariants are
+// computed once per proposal (in fp64, rounded to fp32); the per-visibility
+// work uses the reduced form
+//     ree =  pi (h+h0) Re^2 B(ye)              ime = -2 pi^2 (h-h0) Re^3 U C(ye)
+//     rei =  pi (h+h0-d(h-h0)/Re) Ri^2 B(yi)   imi = -2 pi^2 (h-h0) rq Ri^3 U C(yi)
+// with ye = (2 pi Re)^2 rho^2, yi = rq^2 ye, B(y) = J1(x)/x, C(y) = J2(x)/x^2,
+// which follows from J1' = (J0 - J2)/2 and J0 + J2 = 2 J1/x and removes every
+// 1/rho and the separate rho -> 0 branch of the reference (B(0) = 1/2 and
+// C(0) = 1/8 reproduce its limits, model.cuh:411-416).
+#pragma once
+#include "mhre_device.h"
+
+namespace mhre {
+
+constexpr double D_PI = 3.141592653589793;
+constexpr double UAS2NRAD = D_PI / 180.0 / 3600.0 * 1e3;
+constexpr double LOG2E = 1.4426950408889634;
+// 2 (pi / (2 sqrt(2 ln 2)))^2: FWHM^2 -> Gaussian exponent (model.cuh:372)
+constexpr double GCOEF = 3.5597073019331127;
+
+struct RingInv {
+  float se, rq2;           // ye = se * rho^2, yi = rq2 * ye
+  float kBe, kCe, kBi, kCi;
+  float kd;                // inner-disk shift phase per unit U: 2 pi d
+  float cr, cg;            // ring and Gaussian mixing weights (Zsp folded in)
+  int gauss;               // Gaussian present
+};
+
+// h, h0 and the ring coefficients (model.cuh:366-375) for radii in nrad.
+__device__ inline void ring_coefs(double Rex, double rq, double fade, double d, RingInv* q) {
+  const double Rin = rq * Rex;
+  const double h = (2.0 / D_PI) / ((Rex * Rex - Rin * Rin * (1.0 + d / Rex)) * fade
+                                   + (Rex * Rex - Rin * Rin * (1.0 - d / Rex)));
+  const double h0 = fade * h;
+  q->se = (float)((2.0 * D_PI * Rex) * (2.0 * D_PI * Rex));
+  q->rq2 = (float)(rq * rq);
+  q->kBe = (float)(D_PI * (h + h0) * Rex * Rex);
+  q->kCe = (float)(-2.0 * D_PI * D_PI * (h - h0) * Rex * Rex * Rex);
+  q->kBi = (float)(D_PI * (h + h0 - d * (h - h0) / Rex) * Rin * Rin);
+  q->kCi = (float)(-2.0 * D_PI * D_PI * (h - h0) * rq * Rin * Rin * Rin);
+  q->kd = (float)(2.0 * D_PI * d);
+}
+
+// Ring visibility Fr = Fex - Fin at rotated coordinates (U, V).
+// uniform_small: caller guarantees ye <= 64 on every active lane.
+__device__ inline float2 ring_vis(const RingInv& q, float U, float V, bool any_large) {
+  const float rho2 = fmaf(U, U, V * V);
+  const float ye = q.se * rho2;
+  const float yi = q.rq2 * ye;
+  float Be, Ce, Bi, Ci;
+  if (!any_large) {
+    bessel_BC_poly(ye, &Be, &Ce);
+    bessel_BC_poly(yi, &Bi, &Ci);
+  } else {
+    bessel_BC(ye, &Be, &Ce);
+    bessel_BC(yi, &Bi, &Ci);
+  }
+  const float ree = q.kBe * Be;
+  const float ime = q.kCe * U * Ce;
+  const float rei = q.kBi * Bi;
+  const float imi = q.kCi * U * Ci;
+  float sd, cd;
+  sincos_fast(q.kd * U, &sd, &cd);
+  float2 F;
+  F.x = ree - (rei * cd - imi * sd);
+  F.y = ime - (rei * sd + imi * cd);
+  return F;
+}
+
+// ------------------------------------------------------------ 9 parameters
+struct Sgra9 {
+  static constexpr int NP = 9;
+  struct Inv {
+    RingInv r;
+    float cth, sth;       // rotation by th
+    float gaU, gaV;       // Gaussian exponent (log2 units) per U^2, V^2
+    float kg;             // Gaussian shift phase per unit U: -2 pi (Ri - d)
+  };
+
+  // p[i] = parameter i (Zsp, Re, rq, ecc, fade, gax, aq, gq, th).
+  __device__ static Inv prepare(const float* p) {
+    Inv q;
+    const double Zsp = p[0], Re = p[1], rq = p[2], ecc = p[3], fade = p[4];
+    const double gax = p[5], aq = p[6], gq = p[7], th = p[8];
+    const double Rex = Re * UAS2NRAD;
+    const double Rin = rq * Rex;
+    const double d = ecc * (Rex - Rin);
+    ring_coefs(Rex, rq, fade, d, &q.r);
+    double s, c;
+    sincos(th, &s, &c);
+    q.cth = (float)c;
+    q.sth = (float)s;
+    const bool nog = (p[5] == 0.0f) || (p[6] == 0.0f) || (p[7] == 0.0f);
+    const double gsy = gax * Rex, gsx = aq * gsy;
+    q.gaU = (float)(GCOEF * LOG2E * gsx * gsx);
+    q.gaV = (float)(GCOEF * LOG2E * gsy * gsy);
+    q.kg = (float)(-2.0 * D_PI * (Rin - d));
+    q.r.gauss = nog ? 0 : 1;
+    q.r.cr = (float)(nog ? Zsp : Zsp * (1.0 - gq));
+    q.r.cg = (float)(nog ? 0.0 : Zsp * gq);
+    return q;
+  }
+
+  __device__ static float max_y(const Inv& q, float u, float v) {
+    return q.r.se * fmaf(u, u, v * v);   // rotation preserves rho
+  }
+
+  __device__ static float2 vis(const Inv& q, float u, float v, bool any_large) {
+    const float U = fmaf(u, q.cth, v * q.sth);
+    const float V = fmaf(v, q.cth, -u * q.sth);
+    const float2 F = ring_vis(q.r, U, V, any_large);
+    float2 out;
+    if (q.r.gauss) {
+      const float ga = exp2f(-fmaf(q.gaU, U * U, q.gaV * (V * V)));
+      float sg, cgs;
+      sincos_fast(q.kg * U, &sg, &cgs);
+      const float gw = q.r.cg * ga;
+      out.x = fmaf(q.r.cr, F.x, gw * cgs);
+      out.y = fmaf(q.r.cr, F.y, gw * sg);
+    } else {
+      out.x = q.r.cr * F.x;
+      out.y = q.r.cr * F.y;
+    }
+    return out;
+  }
+};
+
+// ----------------------------------------------------------- 13 parameters
+struct Sgra13 {
+  static constexpr int NP = 13;
+  struct Inv {
+    RingInv r;
+    float cth, sth;       // ring rotation by th
+    float cb, sb;         // Gaussian rotation by beta
+    float gaU, gaV;
+    float kgu, kgv;       // Gaussian phase per unit (u, v): -2 pi gr Re (cos alpha, sin alpha)
+    int gonly;            // gq == 1: Gaussian only
+  };
+
+  // p: Zsp, Re, rq, ecc, fade, gr, gax, aq, gq, alpha, beta, eta, th
+  __device__ static Inv prepare(const float* p) {
+    Inv q;
+    const double Zsp = p[0], Re = p[1], rq = p[2], ecc = p[3], fade = p[4];
+    const double gr = p[5], gax = p[6], aq = p[7], gq = p[8];
+    const double alpha = p[9], beta = p[10], eta = p[11], th = p[12];
+    const double Rex = Re * UAS2NRAD;
+    const double Rin = rq * Rex;
+    const double d = ecc * (Rex - Rin) * cos(eta - th);
+    ring_coefs(Rex, rq, fade, d, &q.r);
+    q.cth = (float)cos(th);
+    q.sth = (float)sin(th);
+    q.cb = (float)cos(beta);
+    q.sb = (float)sin(beta);
+    const bool gauss = (p[6] != 0.0f) && (p[7] != 0.0f) && (p[8] != 0.0f);
+    const double gsy = gax * Rex, gsx = aq * gsy;
+    q.gaU = (float)(GCOEF * LOG2E * gsx * gsx);
+    q.gaV = (float)(GCOEF * LOG2E * gsy * gsy);
+    q.kgu = (float)(-2.0 * D_PI * gr * Rex * cos(alpha));
+    q.kgv = (float)(-2.0 * D_PI * gr * Rex * sin(alpha));
+    q.gonly = (p[8] == 1.0f) ? 1 : 0;
+    q.r.gauss = gauss ? 1 : 0;
+    q.r.cr = (float)(gauss ? Zsp * (1.0 - gq) : Zsp);
+    q.r.cg = (float)(gauss ? Zsp * gq : 0.0);
+    if (q.gonly) { q.r.cr = 0.0f; q.r.cg = (float)(gauss ? Zsp : 0.0); }
+    return q;
+  }
+
+  __device__ static float max_y(const Inv& q, float u, float v) {
+    return q.r.se * fmaf(u, u, v * v);
+  }
+
+  __device__ static float2 vis(const Inv& q, float u, float v, bool any_large) {
+    float2 out = make_float2(0.0f, 0.0f);
+    if (!q.gonly) {
+      const float U = fmaf(u, q.cth, v * q.sth);
+      const float V = fmaf(v, q.cth, -u * q.sth);
+      const float2 F = ring_vis(q.r, U, V, any_large);
+      out.x = q.r.cr * F.x;
+      out.y = q.r.cr * F.y;
+    }
+    if (q.r.gauss) {
+      const float Ub = fmaf(u, q.cb, v * q.sb);
+      const float Vb = fmaf(v, q.cb, -u * q.sb);
+      const float ga = exp2f(-fmaf(q.gaU, Ub * Ub, q.gaV * (Vb * Vb)));
+      float sg, cgs;
+      sincos_fast(fmaf(q.kgu, u, q.kgv * v), &sg, &cgs);
+      const float gw = q.r.cg * ga;
+      out.x = fmaf(gw, cgs, out.x);
+      out.y = fmaf(gw, sg, out.y);
+    }
+    return out;
+  }
+};
+
+}  // namespace mhre

@@ -1,0 +1,34 @@
+// Host-callable launchers of the engine kernels (engine.hip, generic.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "fitdesc.h"
+
+namespace mhre {
+
+struct StepLaunch {
+  const FitDesc* fits;   // device
+  const WorkItem* work;  // device, step work list
+  int nwork;
+  RunClock* clk;         // device
+  int wpb;               // waves per work-group (1, 2, 4)
+  int persistent;        // 1: grid-stride work queue
+  int grid;              // work-groups for the persistent form
+  int qslot;             // RunClock::qhead slot of the persistent form
+  int* tr_dec; float* tr_chi2; int* tr_ex; int tr_it0; int tr_nit;   // optional decision trace
+};
+
+hipError_t launch_init(const FitDesc* fits, const WorkItem* work, int nwork, hipStream_t s);
+hipError_t launch_eval_eht(int model, const FitDesc* fits, const WorkItem* work, int nwork, int outputs,
+                           int respect_flag, hipStream_t s);
+hipError_t launch_step_eht(int model, const StepLaunch& L, hipStream_t s);
+hipError_t launch_eval_quad(int n, const FitDesc* fits, const WorkItem* work, int nwork, hipStream_t s);
+hipError_t launch_step_quad(int n, const StepLaunch& L, hipStream_t s);
+hipError_t launch_exchange(const StepLaunch& L, const WorkItem* xwork, int nx, hipStream_t s);
+
+// generic.hip: the library's compiled-in model.cuh (reference contract)
+hipError_t launch_eval_user(const FitDesc* fits, const WorkItem* work, int nwork, int model_only,
+                            int respect_flag, hipStream_t s);
+hipError_t launch_step_user(const StepLaunch& L, hipStream_t s);
+const char* user_model_name();
+
+}  // namespace mhre

@@ -1,0 +1,723 @@
+// MH-RE engine kernels for gfx950.
+//
+// Reference: src/gpu_mcmc.cu.  The reference launches, per iteration and per
+// parameter, gen_proposal, npass x calc_chi2_terms, spmp_mcmc and
+// adjust_steps, then replica_exchange (:727-757) -- about 46 launches per
+// burn-in iteration with every intermediate in global memory.  Here one launch
+// of the step kernel runs all nprm proposals of an iteration for every chain
+// with the chain state in registers, and one launch of the exchange kernel
+// does the swaps and the output recording:
+//
+//   k_init_chains    init_pcur (:245-267)
+//   k_eval_eht       calc_chi2_terms + calc_chi2 (:273-350) for the native
+//                    ring+Gaussian models; also fills datm/chi2m
+//   k_step_eht       gen_proposal + calc_chi2_terms + spmp_mcmc + adjust_steps
+//                    (:361-526) fused; one wave64 per chain, uv data streamed
+//                    through the wave, model visibilities kept in a per-wave
+//                    LDS ring for the closure phases, chi^2 accumulated in fp64
+//                    per lane and reduced across the wave
+//   k_step_quad      the same for systems of quadratic equations, one lane per
+//                    chain (wave-uniform coefficients)
+//   k_exchange       replica_exchange (:533-626); one wave per sequence, one
+//                    lane per temperature slot
+#include <hip/hip_runtime.h>
+#include "mhre_device.h"
+#include "eht_model.h"
+#include "fitdesc.h"
+#include "engine.h"
+
+namespace mhre {
+
+// ------------------------------------------------------------------ helpers
+
+__device__ inline int pick_with_retry(uint32_t w0, int n, uint32_t c0, uint32_t c1, uint32_t c2,
+                                      uint32_t c3base, uint32_t k0, uint32_t k1) {
+  int k = pick_index(w0, n);
+  for (int r = 1; k >= n && r <= RNG_MAX_RETRY; ++r)
+    k = pick_index(philox4x32_10(c0, c1, c2, c3base + (uint32_t)r, k0, k1).x, n);
+  return k >= n ? n - 1 : k;
+}
+
+// Box-Muller in float64 (same formula as the oracle, so proposals round to
+// the same fp32 value).
+__device__ inline double normal64(uint32_t w1, uint32_t w2) {
+  const double u1 = (double)u01(w1), u2 = (double)u01(w2);
+  return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
+}
+
+__device__ inline int load_clock(const RunClock* clk) {
+  return __atomic_load_n(&clk->it, __ATOMIC_RELAXED);
+}
+
+// ------------------------------------------------------------- init chains
+
+__global__ void k_init_chains(const FitDesc* fits, const WorkItem* work, int nwork) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nwork) return;
+  const WorkItem w = work[g];
+  const FitDesc& f = fits[w.fit];
+  const int ch = w.idx;
+  for (int i = 0; i < f.nprm; ++i) {
+    const int k = f.ivar[i];
+    const uint32_t x = philox4x32_10((uint32_t)ch, RNG_INIT_ITER, (uint32_t)i, RNG_INIT, f.key0, f.key1).x;
+    const double lo = f.pmint[k], hi = f.pmaxt[k];
+    double p = lo + (hi - lo) * (double)u01(x);
+    if (f.pdescr[k] == 2) p = p - 6.283185307179586 * rint(p / 6.283185307179586);
+    f.pcur[(size_t)ch * f.nptot + k] = (float)p;
+  }
+  f.flag[ch] = 1;
+}
+
+// -------------------------------------------------- native chi^2 (one wave)
+
+// Wave-uniform view of one fit's native-kernel inputs; built once per wave
+// from a uniform FitDesc address so every field lives in scalar registers.
+// Global address space for the device pass (plain pointers in the host pass,
+// which never dereferences them): loads become global_load with scalar
+// bases instead of flat loads, which also count against the LDS counter.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GLOBAL __attribute__((address_space(1)))
+#else
+#define GLOBAL
+#endif
+template <class T>
+__device__ inline GLOBAL T* gptr(T* p) { return (GLOBAL T*)p; }
+
+struct EhtView {
+  const GLOBAL float4* vis;
+  const GLOBAL int4* clo;
+  const GLOBAL float4* clow;
+  const GLOBAL int* clo_range;
+  GLOBAL float* datm;
+  GLOBAL float* chi2m;
+  long long ndatm;
+  int nvis, ncph, ntile, ring_ok, ndat, nchain;
+};
+
+__device__ inline EhtView eht_view(const FitDesc& f) {
+  EhtView v;
+  v.vis = gptr(f.vis); v.clo = gptr(f.clo); v.clow = gptr(f.clow); v.clo_range = gptr(f.clo_range);
+  v.datm = gptr(f.datm); v.chi2m = gptr(f.chi2m); v.ndatm = f.ndatm;
+  v.nvis = f.nvis; v.ncph = f.ncph; v.ntile = f.ntile; v.ring_ok = f.ring_ok;
+  v.ndat = f.ndat; v.nchain = f.nchain;
+  return v;
+}
+
+// Orders this wave's LDS ring stores before its later ring loads (and loads
+// before later stores).  LDS operations of one wave execute in order, so
+// only compiler reordering must be prevented; global memory is unaffected.
+__device__ inline void ring_fence() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
+// Closure term from three model visibilities: the residual
+// wrap(obs - (pa + pb - pc)) as arg(e^{i obs} conj(a b conj(c))).
+__device__ inline float closure_term(float2 a, float2 b, float2 e, float4 ow, float* model_cph) {
+  const float abr = a.x * b.x - a.y * b.y, abi = a.x * b.y + a.y * b.x;
+  const float br = abr * e.x + abi * e.y, bi = abi * e.x - abr * e.y;   // a b conj(e)
+  const float xr = ow.x * br + ow.y * bi;
+  const float xi = ow.y * br - ow.x * bi;
+  const float dd = atan2_fast(xi, xr);
+  if (model_cph) *model_cph = atan2_fast(bi, br);
+  return dd * dd * ow.z;
+}
+
+// chi^2 of one chain whose model invariants are q.  Lanes stream 64
+// visibilities per step (the next tile's data and this tile's closure inputs
+// are fetched before the model is evaluated); model visibilities go to a
+// per-wave LDS ring from which the closures of the tile are formed.
+// The fp32 terms are accumulated in fp64 per lane and summed across the wave.
+// OUT: also write datm/chi2m of chain ch in the reference layout.
+template <class M, bool OUT>
+__device__ double eht_chi2_wave(const EhtView& v, const typename M::Inv& q, float2* ring, int ch) {
+  const int lane = lane_id();
+  const bool clos = v.ncph > 0 && v.ring_ok;
+  const bool out = OUT && v.datm != nullptr;
+  const int nvr = v.ndat - v.ncph;   // visibilities with an amplitude datum
+  const int lastv = v.nvis - 1;
+  const int lastc = v.ncph > 0 ? v.ncph - 1 : 0;
+  double acc = 0.0;
+  // All loads below are unconditional with clamped (in-bounds) indices so the
+  // compiler can keep them in flight across the model evaluation.
+  float4 dn = v.vis[min(lane, lastv)];
+  int c0 = 0;
+  int c1 = clos ? v.clo_range[1] : 0;
+  for (int t = 0; t < v.ntile; ++t) {
+    const int i = t * TILE + lane;
+    const float4 d = dn;
+    dn = v.vis[min(i + TILE, lastv)];
+    const int c1n = clos ? v.clo_range[min(t + 2, v.ntile)] : 0;
+    const int cf = c0 + lane;
+    int4 ix = make_int4(0, 0, 0, 0);
+    float4 ow = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if (clos) {
+      ix = v.clo[min(cf, lastc)];
+      ow = v.clow[min(cf, lastc)];
+    }
+    const bool live = i < v.nvis;
+    const bool any_large = __any(live && M::max_y(q, d.x, d.y) > 64.0f);
+    const float2 V = M::vis(q, d.x, d.y, any_large);
+    const float amp = __builtin_amdgcn_sqrtf(fmaf(V.x, V.x, V.y * V.y));
+    const float r = d.z - amp;
+    const float term = r * r * d.w;
+    if (live) acc += (double)term;
+    if (out && live && i < nvr) {
+      const size_t imd = (size_t)ch * v.ndat + i;
+      v.datm[imd] = amp;
+      v.chi2m[imd] = term;
+      const long long tail = (long long)v.nchain * v.ndat + (long long)ch * nvr + i;
+      if (tail < v.ndatm) v.datm[tail] = atan2_fast(V.y, V.x);
+    }
+    // Lanes past nvis write slots of indices four tiles back, which no closure
+    // of this tile reads (closures span at most RING - TILE indices).
+    if (clos) ring[i & (RING - 1)] = V;
+    if (clos && c1 > c0) {
+      ring_fence();
+      for (int c = cf; c < c1; c += TILE) {
+        if (c != cf) {
+          ix = v.clo[c];
+          ow = v.clow[c];
+        }
+        float mc;
+        const float cterm = closure_term(ring[ix.x & (RING - 1)], ring[ix.y & (RING - 1)],
+                                         ring[ix.z & (RING - 1)], ow, out ? &mc : nullptr);
+        acc += (double)cterm;
+        if (out) {
+          const size_t imd = (size_t)ch * v.ndat + ix.w;
+          v.datm[imd] = mc;
+          v.chi2m[imd] = cterm;
+        }
+      }
+      ring_fence();
+    }
+    c0 = c1;
+    c1 = c1n;
+  }
+  if (v.ncph && !v.ring_ok) {
+    // General closure layout: recompute the three model visibilities.
+    for (int c = lane; c < v.ncph; c += TILE) {
+      const int4 ix = v.clo[c];
+      const float4 ow = v.clow[c];
+      const float4 da = v.vis[ix.x], db = v.vis[ix.y], de = v.vis[ix.z];
+      float mc;
+      const float term = closure_term(M::vis(q, da.x, da.y, true), M::vis(q, db.x, db.y, true),
+                                      M::vis(q, de.x, de.y, true), ow, out ? &mc : nullptr);
+      acc += (double)term;
+      if (out) {
+        const size_t imd = (size_t)ch * v.ndat + ix.w;
+        v.datm[imd] = mc;
+        v.chi2m[imd] = term;
+      }
+    }
+  }
+  return wave_sum(acc);
+}
+
+// Gather the chain's parameters (lane j holds parameter j) into a uniform
+// array and build the model invariants.
+template <class M>
+__device__ inline typename M::Inv prepare_from_lanes(float pl) {
+  float p[M::NP];
+#pragma unroll
+  for (int j = 0; j < M::NP; ++j) p[j] = bcast(pl, j);
+  return M::prepare(p);
+}
+
+// ------------------------------------------------------------ eval kernel
+
+template <class M, bool OUT>
+__global__ void __launch_bounds__(256) k_eval_eht(const FitDesc* fits, const WorkItem* work, int nwork,
+                                                  int respect_flag) {
+  __shared__ float2 ring_all[4][RING];
+  const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int wv = blockIdx.x * 4 + wid;
+  if (wv >= nwork) return;
+  float2* ring = ring_all[wid];
+  const WorkItem w = work[wv];
+  const FitDesc& f = fits[w.fit];
+  const int ch = w.idx;
+  const int lane = lane_id();
+  if (respect_flag && f.flag[ch] == 0) {
+    double s = 0.0;
+    if (f.chi2m)
+      for (int i = lane; i < f.ndat; i += TILE) s += (double)f.chi2m[(size_t)ch * f.ndat + i];
+    s = wave_sum(s);
+    if (lane == 0) f.chi2c[ch] = (float)s;
+    return;
+  }
+  const float pl = lane < f.nptot ? f.pcur[(size_t)ch * f.nptot + lane] : 0.0f;
+  const typename M::Inv q = prepare_from_lanes<M>(pl);
+  const double c2 = eht_chi2_wave<M, OUT>(eht_view(f), q, ring, ch);
+  if (lane == 0) f.chi2c[ch] = (float)c2;
+}
+
+// ------------------------------------------------------------ step kernel
+
+struct TraceBuf {
+  int* dec;      // [nit][nprm][nchain] : k | inprior << 8 | accept << 9
+  float* chi2;   // [nit][nprm][nchain]
+  int* ex;       // [nit][nbeta][nseq] : k | accept << 16
+  int it0, nit;
+};
+
+template <class M>
+__device__ void step_one_chain(const FitDesc& f, int ch, int it, float2* ring, const TraceBuf& tr) {
+  const int lane = lane_id();
+  const int C = f.nchain, P = f.nprm, N = f.nptot;
+  const int ibeta = ch / f.nseq;
+  const bool burn = it < f.nburn;
+
+  // lane-distributed chain state and constants
+  float pl = lane < N ? f.pcur[(size_t)ch * N + lane] : 0.0f;
+  const int pdl = lane < N ? f.pdescr[lane] : 0;
+  const float pminl = lane < N ? f.pmint[lane] : 0.0f;
+  const float pmaxl = lane < N ? f.pmaxt[lane] : 0.0f;
+  const int ivl = lane < P ? f.ivar[lane] : 0;
+  float stp = lane < P ? f.pstp[(size_t)lane * C + ch] : 0.0f;
+  int cnt = lane < P ? f.n_cnt[(size_t)lane * C + ch] : 0;
+  int nac = lane < P ? f.n_acpt[(size_t)lane * C + ch] : 0;
+  int sh = (burn && lane < P) ? f.sumh[(size_t)lane * C + ch] : 0;
+  const double beta = (double)f.beta[ibeta];
+  float chi2c = f.chi2c[ch];
+  int flag = 0, tlast = 0;
+  float told = 0.0f;
+  unsigned nev = 0;
+  const EhtView view = eht_view(f);
+
+  for (int s = 0; s < P; ++s) {
+    const U4 r = philox4x32_10((uint32_t)ch, (uint32_t)it, (uint32_t)s, 0u, f.key0, f.key1);
+    const int k = pick_with_retry(r.x, P, (uint32_t)ch, (uint32_t)it, (uint32_t)s, 0u, f.key0, f.key1);
+    const int t = bcast(ivl, k);
+    // history entry that this proposal will overwrite (issued early)
+    int hold = 0;
+    size_t hidx = 0;
+    if (burn && lane == k) {
+      hidx = ((size_t)P * (size_t)(cnt % f.nadj) + (size_t)k) * C + ch;
+      hold = f.n_hist[hidx];
+    }
+    const float old = bcast(pl, t);
+    const float sd = bcast(stp, k);
+    double pd = (double)old + (double)sd * normal64(r.y, r.z);
+    if (bcast(pdl, t) == 2) pd = pd - 6.283185307179586 * rint(pd / 6.283185307179586);
+    const float pn = (float)pd;
+    const bool inprior = !(pn < bcast(pminl, t) || pn > bcast(pmaxl, t));
+    int acc = 0;
+    double c2 = 0.0;
+    if (inprior) {
+      if (lane == t) pl = pn;
+      const typename M::Inv q = prepare_from_lanes<M>(pl);
+      c2 = eht_chi2_wave<M, false>(view, q, ring, ch);
+      ++nev;
+      const double x = -0.5 * beta * (c2 - (double)chi2c);
+      const double lu = log((double)u01(r.w));
+      if (lu <= x) {
+        acc = 1;
+        chi2c = (float)c2;
+        if (lane == k) ++nac;
+      } else if (lane == t) {
+        pl = old;
+      }
+    }
+    flag = acc;
+    told = old;
+    tlast = t;
+    if (burn && lane == k) {
+      f.n_hist[hidx] = acc;
+      sh += acc - hold;
+      const int den = cnt < f.nadj ? cnt : f.nadj;
+      const float rate = (float)sh / (float)den;
+      stp = rate < f.adapt_target ? (float)((double)stp / 1.01) : (float)((double)stp * 1.01);
+      ++cnt;
+    }
+    if (tr.dec && it >= tr.it0 && it < tr.it0 + tr.nit && lane == 0) {
+      const size_t ti = ((size_t)(it - tr.it0) * P + s) * C + ch;
+      tr.dec[ti] = k | ((inprior ? 1 : 0) << 8) | (acc << 9);
+      tr.chi2[ti] = inprior ? (float)c2 : 0.0f;
+    }
+  }
+
+  if (lane < N) f.pcur[(size_t)ch * N + lane] = pl;
+  if (lane < P) {
+    f.pstp[(size_t)lane * C + ch] = stp;
+    f.n_cnt[(size_t)lane * C + ch] = cnt;
+    f.n_acpt[(size_t)lane * C + ch] = nac;
+    if (burn) f.sumh[(size_t)lane * C + ch] = sh;
+  }
+  if (lane == 0) {
+    f.chi2c[ch] = chi2c;
+    f.flag[ch] = flag;
+    f.ptent[ch] = told;
+    f.ptentn[ch] = tlast;
+    if (f.nevals) {
+      atomicAdd(&f.nevals[0], (unsigned long long)nev);
+      atomicAdd(&f.nevals[1], (unsigned long long)P);
+    }
+  }
+}
+
+// Next work item of the persistent step kernel: one atomic per wave (lane 0),
+// broadcast to the wave.  Every wave stops once the queue is exhausted.
+__device__ inline int grab_item(RunClock* clk, int qslot) {
+  const int leader = __builtin_amdgcn_readfirstlane(lane_id());
+  int v = 0;
+  if (lane_id() == leader) v = atomicAdd(&clk->qhead[qslot], 1);
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+template <class M, int WPB, bool PERSIST>
+__global__ void __launch_bounds__(64 * WPB) k_step_eht(const FitDesc* fits, const WorkItem* work, int nwork,
+                                                       RunClock* clk, TraceBuf tr, int qslot) {
+  __shared__ float2 ring_all[WPB][RING];
+  const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  float2* ring = ring_all[wid];
+  const int it = __builtin_amdgcn_readfirstlane(load_clock(clk));
+  if (!PERSIST) {
+    const int wv = blockIdx.x * WPB + wid;
+    if (wv >= nwork) return;
+    const WorkItem w = work[wv];
+    const FitDesc& f = fits[w.fit];
+    if (it >= f.nburn + f.niter) return;
+    step_one_chain<M>(f, w.idx, it, ring, tr);
+  } else {
+    for (int wv = grab_item(clk, qslot); wv < nwork; wv = grab_item(clk, qslot)) {
+      const WorkItem w = work[wv];
+      const FitDesc& f = fits[w.fit];
+      if (it < f.nburn + f.niter) step_one_chain<M>(f, w.idx, it, ring, tr);
+    }
+  }
+}
+
+// --------------------------------------------------- quadratic systems (TPC)
+
+// Per equation e the packed coefficients are qcoef[fit][e][QS]:
+//   n(n+1)/2 symmetric products (i <= j, Q_ij + Q_ji off the diagonal),
+//   n linear terms b_e, then rhs = dat[e] and the weight std2r[e].
+template <int NQ>
+struct QuadSys {
+  static constexpr int NS = NQ * (NQ + 1) / 2;
+  static constexpr int QS = NS + NQ + 2;
+  __device__ static double chi2(const float* __restrict__ qc, int neq, const float (&x)[NQ]) {
+    float xx[NS];
+    int m = 0;
+#pragma unroll
+    for (int i = 0; i < NQ; ++i)
+#pragma unroll
+      for (int j = i; j < NQ; ++j) xx[m++] = x[i] * x[j];
+    double acc = 0.0;
+    for (int e = 0; e < neq; ++e) {
+      const float* c = qc + (size_t)e * QS;
+      float lhs = 0.0f;
+#pragma unroll
+      for (int i = 0; i < NS; ++i) lhs = fmaf(c[i], xx[i], lhs);
+#pragma unroll
+      for (int i = 0; i < NQ; ++i) lhs = fmaf(c[NS + i], x[i], lhs);
+      const float r = c[NS + NQ] - lhs;
+      acc += (double)(r * r * c[NS + NQ + 1]);
+    }
+    return acc;
+  }
+};
+
+template <int NQ>
+__global__ void __launch_bounds__(256) k_eval_quad(const FitDesc* fits, const WorkItem* work, int nwork) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nwork) return;
+  const WorkItem w = work[g];
+  const FitDesc& f = fits[w.fit];
+  float x[NQ];
+#pragma unroll
+  for (int i = 0; i < NQ; ++i) x[i] = f.pcur[(size_t)w.idx * NQ + i];
+  f.chi2c[w.idx] = (float)QuadSys<NQ>::chi2(f.qcoef, f.quad_neq, x);
+}
+
+// One lane per chain; all chains of a wave belong to one fit (the host pads
+// every fit's chain list to whole waves), so the coefficients are uniform.
+template <int NQ>
+__global__ void __launch_bounds__(256) k_step_quad(const FitDesc* fits, const WorkItem* work, int nwork,
+                                                   RunClock* clk, TraceBuf tr) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nwork) return;
+  const WorkItem w = work[g];
+  if (w.idx < 0) return;   // padding lane
+  const FitDesc& f = fits[w.fit];
+  const int it = load_clock(clk);
+  if (it >= f.nburn + f.niter) return;
+  const int ch = w.idx, C = f.nchain, P = f.nprm;
+  const int ibeta = ch / f.nseq;
+  const bool burn = it < f.nburn;
+  float x[NQ];
+#pragma unroll
+  for (int i = 0; i < NQ; ++i) x[i] = f.pcur[(size_t)ch * NQ + i];
+  const double beta = (double)f.beta[ibeta];
+  float chi2c = f.chi2c[ch];
+  int flag = 0, tlast = 0;
+  float told = 0.0f;
+  unsigned nev = 0;
+  for (int s = 0; s < P; ++s) {
+    const U4 r = philox4x32_10((uint32_t)ch, (uint32_t)it, (uint32_t)s, 0u, f.key0, f.key1);
+    const int k = pick_with_retry(r.x, P, (uint32_t)ch, (uint32_t)it, (uint32_t)s, 0u, f.key0, f.key1);
+    const int t = f.ivar[k];
+    const size_t pk = (size_t)k * C + ch;
+    float old = 0.0f;
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) old = (i == t) ? x[i] : old;
+    const float sd = f.pstp[pk];
+    double pd = (double)old + (double)sd * normal64(r.y, r.z);
+    if (f.pdescr[t] == 2) pd = pd - 6.283185307179586 * rint(pd / 6.283185307179586);
+    const float pn = (float)pd;
+    const bool inprior = !(pn < f.pmint[t] || pn > f.pmaxt[t]);
+    int acc = 0;
+    double c2 = 0.0;
+    if (inprior) {
+#pragma unroll
+      for (int i = 0; i < NQ; ++i) x[i] = (i == t) ? pn : x[i];
+      c2 = QuadSys<NQ>::chi2(f.qcoef, f.quad_neq, x);
+      ++nev;
+      const double xx = -0.5 * beta * (c2 - (double)chi2c);
+      if (log((double)u01(r.w)) <= xx) {
+        acc = 1;
+        chi2c = (float)c2;
+        f.n_acpt[pk] += 1;
+      } else {
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) x[i] = (i == t) ? old : x[i];
+      }
+    }
+    flag = acc;
+    told = old;
+    tlast = t;
+    if (burn) {
+      const int cnt = f.n_cnt[pk];
+      const size_t hidx = ((size_t)P * (size_t)(cnt % f.nadj) + (size_t)k) * C + ch;
+      const int hold = f.n_hist[hidx];
+      f.n_hist[hidx] = acc;
+      const int sh = f.sumh[pk] + acc - hold;
+      f.sumh[pk] = sh;
+      const int den = cnt < f.nadj ? cnt : f.nadj;
+      const float rate = (float)sh / (float)den;
+      const float st = f.pstp[pk];
+      f.pstp[pk] = rate < f.adapt_target ? (float)((double)st / 1.01) : (float)((double)st * 1.01);
+      f.n_cnt[pk] = cnt + 1;
+    }
+    if (tr.dec && it >= tr.it0 && it < tr.it0 + tr.nit) {
+      const size_t ti = ((size_t)(it - tr.it0) * P + s) * C + ch;
+      tr.dec[ti] = k | ((inprior ? 1 : 0) << 8) | (acc << 9);
+      tr.chi2[ti] = inprior ? (float)c2 : 0.0f;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NQ; ++i) f.pcur[(size_t)ch * NQ + i] = x[i];
+  f.chi2c[ch] = chi2c;
+  f.flag[ch] = flag;
+  f.ptent[ch] = told;
+  f.ptentn[ch] = tlast;
+  if (f.nevals) {
+    atomicAdd(&f.nevals[0], (unsigned long long)nev);
+    atomicAdd(&f.nevals[1], (unsigned long long)P);
+  }
+}
+
+// ---------------------------------------------------------------- exchange
+
+__device__ void exchange_wave(const FitDesc& f, int iseq, int it, bool burn, const TraceBuf& tr) {
+  const int lane = lane_id();
+  const int B = f.nbeta, S = f.nseq, N = f.nptot;
+  const bool slot = lane < B;
+  const size_t me = (size_t)lane * S + iseq;
+  float c2 = slot ? f.chi2c[me] : 0.0f;
+  int tc = slot ? f.tcur[me] : 0;
+  int perm = lane;
+  int nex = 0;
+  if (B > 1) {
+    for (int a = 0; a < B; ++a) {
+      const U4 r = philox4x32_10((uint32_t)iseq, (uint32_t)it, (uint32_t)a, RNG_EXCH, f.key0, f.key1);
+      const int k = pick_with_retry(r.x, B - 1, (uint32_t)iseq, (uint32_t)it, (uint32_t)a, RNG_EXCH,
+                                    f.key0, f.key1);
+      const float ck = bcast(c2, k), ck1 = bcast(c2, k + 1);
+      const double x = -0.5 * ((double)f.beta[k + 1] - (double)f.beta[k]) * ((double)ck - (double)ck1);
+      const bool acc = log((double)u01(r.y)) <= x;
+      if (acc) {
+        const int tk = bcast(tc, k), tk1 = bcast(tc, k + 1);
+        const int pk = bcast(perm, k), pk1 = bcast(perm, k + 1);
+        if (lane == k) { c2 = ck1; tc = tk1; perm = pk1; if (!burn) ++nex; }
+        if (lane == k + 1) { c2 = ck; tc = tk; perm = pk; }
+      }
+      if (tr.ex && lane == 0 && it >= tr.it0 && it < tr.it0 + tr.nit)
+        tr.ex[((size_t)(it - tr.it0) * B + a) * S + iseq] = k | ((acc ? 1 : 0) << 16);
+    }
+  }
+  // move parameter rows: slot lane receives the row that started in slot perm
+  float row[MAXP];
+#pragma unroll
+  for (int j = 0; j < MAXP; ++j)
+    row[j] = (slot && j < N) ? f.pcur[((size_t)perm * S + iseq) * N + j] : 0.0f;
+  if (slot && perm != lane) {
+#pragma unroll
+    for (int j = 0; j < MAXP; ++j)
+      if (j < N) f.pcur[me * N + j] = row[j];
+  }
+  if (slot) {
+    f.chi2c[me] = c2;
+    f.tcur[me] = tc;
+    if (!burn && nex) f.n_exch[me] += nex;
+  }
+  if (!burn) {
+    const int itr = it - f.nburn;
+    if (slot) {
+#pragma unroll
+      for (int j = 0; j < MAXP; ++j) {
+        if (j < N) {
+          const int ip = f.invar[j];
+          if (ip >= 0) f.pout[(((size_t)B * ip + lane) * S + iseq) * f.niter + itr] = row[j];
+        }
+      }
+      f.tout[me * f.niter + itr] = tc;
+    }
+    if (lane == 0) f.chi2[(size_t)iseq * f.niter + itr] = c2;
+  }
+}
+
+// nbeta > 64: the reference's serial loop on one lane (gpu_mcmc.cu:548-624).
+__device__ void exchange_serial(const FitDesc& f, int iseq, int it, bool burn, const TraceBuf& tr) {
+  if (lane_id() != 0) return;
+  const int B = f.nbeta, S = f.nseq, N = f.nptot;
+  for (int a = 0; a < B; ++a) {
+    const U4 r = philox4x32_10((uint32_t)iseq, (uint32_t)it, (uint32_t)a, RNG_EXCH, f.key0, f.key1);
+    const int k = pick_with_retry(r.x, B - 1, (uint32_t)iseq, (uint32_t)it, (uint32_t)a, RNG_EXCH,
+                                  f.key0, f.key1);
+    const size_t k0 = (size_t)k * S + iseq, k1 = (size_t)(k + 1) * S + iseq;
+    const double x = -0.5 * ((double)f.beta[k + 1] - (double)f.beta[k]) *
+                     ((double)f.chi2c[k0] - (double)f.chi2c[k1]);
+    const bool acc = log((double)u01(r.y)) <= x;
+    if (acc) {
+      int ti = f.tcur[k0]; f.tcur[k0] = f.tcur[k1]; f.tcur[k1] = ti;
+      for (int j = 0; j < N; ++j) {
+        float tmp = f.pcur[k0 * N + j]; f.pcur[k0 * N + j] = f.pcur[k1 * N + j]; f.pcur[k1 * N + j] = tmp;
+      }
+      float tc = f.chi2c[k0]; f.chi2c[k0] = f.chi2c[k1]; f.chi2c[k1] = tc;
+      if (!burn) f.n_exch[k0] += 1;
+    }
+    if (tr.ex && it >= tr.it0 && it < tr.it0 + tr.nit)
+      tr.ex[((size_t)(it - tr.it0) * B + a) * S + iseq] = k | ((acc ? 1 : 0) << 16);
+  }
+  if (!burn) {
+    const int itr = it - f.nburn;
+    for (int ib = 0; ib < B; ++ib) {
+      const size_t me = (size_t)ib * S + iseq;
+      for (int j = 0; j < N; ++j) {
+        const int ip = f.invar[j];
+        if (ip >= 0) f.pout[(((size_t)B * ip + ib) * S + iseq) * f.niter + itr] = f.pcur[me * N + j];
+      }
+      f.tout[me * f.niter + itr] = f.tcur[me];
+    }
+    f.chi2[(size_t)iseq * f.niter + itr] = f.chi2c[iseq];
+  }
+}
+
+__global__ void __launch_bounds__(64) k_exchange(const FitDesc* fits, const WorkItem* xwork, int nx,
+                                                 RunClock* clk, TraceBuf tr) {
+  const int it = load_clock(clk);
+  const WorkItem w = xwork[blockIdx.x];
+  const FitDesc& f = fits[w.fit];
+  if (it < f.nburn + f.niter) {
+    const bool burn = it < f.nburn;
+    if (f.nbeta <= 64) exchange_wave(f, w.idx, it, burn, tr);
+    else exchange_serial(f, w.idx, it, burn, tr);
+  }
+  // advance the iteration clock once every work-group of this launch is done
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const int v = atomicAdd(&clk->done, 1);
+    if (v == (int)gridDim.x - 1) {
+      atomicExch(&clk->done, 0);
+      for (int q = 0; q < NQSLOT; ++q) atomicExch(&clk->qhead[q], 0);
+      __threadfence();
+      atomicAdd(&clk->it, 1);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- launchers
+
+static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
+
+hipError_t launch_init(const FitDesc* fits, const WorkItem* work, int nwork, hipStream_t s) {
+  if (nwork <= 0) return hipSuccess;
+  k_init_chains<<<cdiv(nwork, 256), 256, 0, s>>>(fits, work, nwork);
+  return hipGetLastError();
+}
+
+hipError_t launch_eval_eht(int model, const FitDesc* fits, const WorkItem* work, int nwork, int outputs,
+                           int respect_flag, hipStream_t s) {
+  if (nwork <= 0) return hipSuccess;
+  const int g = cdiv(nwork, 4);
+  if (model == MODEL_SGRA9) {
+    if (outputs) k_eval_eht<Sgra9, true><<<g, 256, 0, s>>>(fits, work, nwork, respect_flag);
+    else k_eval_eht<Sgra9, false><<<g, 256, 0, s>>>(fits, work, nwork, respect_flag);
+  } else {
+    if (outputs) k_eval_eht<Sgra13, true><<<g, 256, 0, s>>>(fits, work, nwork, respect_flag);
+    else k_eval_eht<Sgra13, false><<<g, 256, 0, s>>>(fits, work, nwork, respect_flag);
+  }
+  return hipGetLastError();
+}
+
+template <class M>
+static void launch_step_m(const StepLaunch& L, hipStream_t s) {
+  const TraceBuf tr{L.tr_dec, L.tr_chi2, L.tr_ex, L.tr_it0, L.tr_nit};
+  if (L.persistent) {
+    const int g = L.grid;
+    switch (L.wpb) {
+      case 1: k_step_eht<M, 1, true><<<g, 64, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr, L.qslot); break;
+      case 2: k_step_eht<M, 2, true><<<g, 128, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr, L.qslot); break;
+      default: k_step_eht<M, 4, true><<<g, 256, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr, L.qslot); break;
+    }
+  } else {
+    switch (L.wpb) {
+      case 1: k_step_eht<M, 1, false><<<L.nwork, 64, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr, L.qslot); break;
+      case 2: k_step_eht<M, 2, false><<<cdiv(L.nwork, 2), 128, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr, L.qslot); break;
+      default: k_step_eht<M, 4, false><<<cdiv(L.nwork, 4), 256, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr, L.qslot); break;
+    }
+  }
+}
+
+hipError_t launch_step_eht(int model, const StepLaunch& L, hipStream_t s) {
+  if (L.nwork <= 0) return hipSuccess;
+  if (model == MODEL_SGRA9) launch_step_m<Sgra9>(L, s);
+  else launch_step_m<Sgra13>(L, s);
+  return hipGetLastError();
+}
+
+hipError_t launch_eval_quad(int n, const FitDesc* fits, const WorkItem* work, int nwork, hipStream_t s) {
+  if (nwork <= 0) return hipSuccess;
+  switch (n) {
+    case 2: k_eval_quad<2><<<cdiv(nwork, 256), 256, 0, s>>>(fits, work, nwork); break;
+    case 4: k_eval_quad<4><<<cdiv(nwork, 256), 256, 0, s>>>(fits, work, nwork); break;
+    case 8: k_eval_quad<8><<<cdiv(nwork, 256), 256, 0, s>>>(fits, work, nwork); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_step_quad(int n, const StepLaunch& L, hipStream_t s) {
+  if (L.nwork <= 0) return hipSuccess;
+  const TraceBuf tr{L.tr_dec, L.tr_chi2, L.tr_ex, L.tr_it0, L.tr_nit};
+  const int g = cdiv(L.nwork, 256);
+  switch (n) {
+    case 2: k_step_quad<2><<<g, 256, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr); break;
+    case 4: k_step_quad<4><<<g, 256, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr); break;
+    case 8: k_step_quad<8><<<g, 256, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_exchange(const StepLaunch& L, const WorkItem* xwork, int nx, hipStream_t s) {
+  if (nx <= 0) return hipSuccess;
+  const TraceBuf tr{L.tr_dec, L.tr_chi2, L.tr_ex, L.tr_it0, L.tr_nit};
+  k_exchange<<<nx, 64, 0, s>>>(L.fits, xwork, nx, L.clk, tr);
+  return hipGetLastError();
+}
+
+}  // namespace mhre

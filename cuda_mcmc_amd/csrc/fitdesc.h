@@ -1,0 +1,89 @@
+// Device-resident description of one MH-RE fit.  A batch is an array of
+// these; a plain (reference-style) run is a batch of one.
+//
+// All chain-state arrays use the reference's layouts (src/mcmc_interf.pyx:
+// 232-263) so they can be copied to and from the caller's buffers verbatim:
+//   pcur[nbeta][nseq][nptot], chi2c/flag/tcur/ptent/ptentn[nbeta][nseq],
+//   pstp/n_cnt/n_acpt[nprm][nbeta][nseq], n_hist[nadj][nprm][nbeta][nseq],
+//   n_exch[nbeta][nseq], pout[nprm][nbeta][nseq][niter],
+//   tout[nbeta][nseq][niter], chi2[nseq][niter].
+// Chain index ist = nseq * ibeta + iseq (src/gpu_mcmc.cu:247).
+#pragma once
+#include <stdint.h>
+
+class CCalcModel;
+
+namespace mhre {
+
+enum ModelKind : int {
+  MODEL_SGRA9 = 1,    // 9-parameter ring + Gaussian (native fused kernel)
+  MODEL_SGRA13 = 2,   // 13-parameter ring + Gaussian (native fused kernel)
+  MODEL_USER = 10,    // the library's compiled-in model.cuh (generic path)
+  MODEL_QUADSYS = 20, // systems of quadratic equations (native, thread per chain)
+};
+
+constexpr int MAXP = 32;       // max nptot handled by the native kernels
+constexpr int RING = 256;      // per-wave LDS ring of model visibilities
+constexpr int TILE = 64;       // visibilities per wave step
+
+struct FitDesc {
+  // ---- sizes and options
+  int model;          // ModelKind
+  int imodel;         // value handed to the user's model() as mc->imodel
+  int nbeta, nseq, nptot, nprm, nadj, npass;
+  int ndat;           // number of chi^2 terms
+  int nvis;           // visibilities evaluated by the native kernel (incl. cpExt pseudo-vis)
+  int ncph;           // closure phases
+  int ntile;          // ceil(nvis / TILE)
+  int ring_ok;        // every closure lies within one ring window
+  int nburn, niter;
+  int nchain;         // nbeta * nseq
+  uint32_t key0, key1;
+  float adapt_target; // acceptance-rate threshold of the step adaptation (reference: 0.30)
+  int quad_n, quad_neq;  // MODEL_QUADSYS sizes
+  long long ndatm;       // length of datm (head nchain*ndat, then the phase tail)
+
+  // ---- static inputs (device pointers)
+  const float4* vis;       // [nvis] (u, v, amp, 1/sigma^2) for the native kernel
+  const int4* clo;         // [ncph] vis indices (i, j, k, datum index) sorted by ring tile
+  const float4* clow;      // [ncph] (cos obs, sin obs, 1/sigma^2, 0)
+  const int* clo_range;    // [ntile + 1] closures handled after each tile
+  const float* qcoef;      // MODEL_QUADSYS packed coefficients
+  const float* coor; const float* dat; const float* std2r;
+  const int* icoor; const int* idat;
+  const int* pdescr; const int* ivar; const int* invar;
+  const float* pmint; const float* pmaxt; const float* beta;
+
+  // ---- chain state
+  float* pcur; float* chi2c; float* pstp;
+  int* n_cnt; int* n_acpt; int* n_hist; int* n_exch; int* sumh;
+  int* tcur; int* flag; float* ptent; int* ptentn;
+
+  // ---- outputs
+  float* pout; int* tout; float* chi2;
+
+  // ---- workspaces
+  float* datm; float* chi2m;     // reference layout (generic path and eval outputs)
+  float2* vscratch;              // [nchain][nvis] model visibilities when !ring_ok
+  CCalcModel* mc;                // device CCalcModel for the user's model()
+
+  // ---- counters
+  unsigned long long* nevals;    // [2]: performed chi^2 evaluations, proposals
+  int* errflag;                  // first negative model() return code
+};
+
+// Iteration clock shared by the step and exchange kernels so that a cycle of
+// launches can be captured once in a hipGraph and replayed.
+constexpr int NQSLOT = 6;
+struct RunClock {
+  int it;              // global iteration index (burn-in first, then sampling)
+  int done;            // exchange work-groups finished in the current iteration
+  int qhead[NQSLOT];   // work-queue heads of the persistent step kernels
+};
+
+struct WorkItem {
+  int fit;
+  int idx;    // chain (step) or sequence (exchange) within the fit
+};
+
+}  // namespace mhre

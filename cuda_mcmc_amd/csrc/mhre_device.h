@@ -1,0 +1,167 @@
+// Device-side building blocks of the MH-RE engine for gfx950 (CDNA4).
+//
+//  * Philox4x32-10 counter RNG (Salmon et al., SC'11) and the engine's draws:
+//    uniform in (0,1], integer pick with redraw, Box-Muller normal.  The
+//    stream layout (counter = chain, iteration, substep, purpose) replaces the
+//    reference's per-thread cuRAND XORWOW states (src/gpu_mcmc.cu:228-239).
+//  * fp32 special functions used by the visibility model, written from their
+//    mathematical definitions: Cody-Waite reduced sincos with Taylor kernels,
+//    atan2 from a Chebyshev-fitted odd polynomial, and the Bessel-derived
+//    B(y) = J1(x)/x and C(y) = J2(x)/x^2 (y = x^2) from Chebyshev-fitted
+//    polynomials on x <= 8 and Hankel's asymptotic expansion beyond.
+//  * wave64 helpers (lane broadcast, cross-lane sums).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "approx_coeffs.h"
+
+namespace mhre {
+
+constexpr uint32_t PHILOX_M0 = 0xD2511F53u;
+constexpr uint32_t PHILOX_M1 = 0xCD9E8D57u;
+constexpr uint32_t PHILOX_W0 = 0x9E3779B9u;
+constexpr uint32_t PHILOX_W1 = 0xBB67AE85u;
+
+constexpr uint32_t RNG_EXCH = 0x40000000u;
+constexpr uint32_t RNG_INIT = 0x80000000u;
+constexpr uint32_t RNG_INIT_ITER = 0xFFFFFFFFu;
+constexpr int RNG_MAX_RETRY = 8;
+
+struct U4 { uint32_t x, y, z, w; };
+
+__host__ __device__ inline U4 philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                            uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r) { k0 += PHILOX_W0; k1 += PHILOX_W1; }
+    const uint64_t p0 = (uint64_t)PHILOX_M0 * c0;
+    const uint64_t p1 = (uint64_t)PHILOX_M1 * c2;
+    const uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+    const uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+    c1 = (uint32_t)p1;
+    c3 = (uint32_t)p0;
+    c0 = n0;
+    c2 = n2;
+  }
+  return U4{c0, c1, c2, c3};
+}
+
+// Uniform in (0, 1]: ((w >> 8) + 1) * 2^-24, exact in fp32.
+__host__ __device__ inline float u01(uint32_t w) { return (float)((w >> 8) + 1u) * 5.9604644775390625e-8f; }
+
+// floor(n * u) with u = (w + 1) * 2^-32; equals n only when w == 2^32 - 1.
+__host__ __device__ inline int pick_index(uint32_t w, int n) {
+  return (int)(((uint64_t)(uint32_t)n * ((uint64_t)w + 1ull)) >> 32);
+}
+
+__device__ inline float box_muller(uint32_t w1, uint32_t w2) {
+  const float u1 = u01(w1), u2 = u01(w2);
+  return sqrtf(-2.0f * logf(u1)) * cospif(2.0f * u2);
+}
+
+// Angle wrap into [-pi, pi] in float64 (shared bit-for-bit with the oracle).
+__host__ __device__ inline float wrap_angle(float p) {
+  const double two_pi = 6.283185307179586;
+  const double d = (double)p;
+  return (float)(d - two_pi * rint(d / two_pi));
+}
+
+// ---------------------------------------------------------------- fp32 math
+
+__device__ inline float poly_eval(const float* c, int n, float t) {
+  float acc = c[n - 1];
+  for (int k = n - 2; k >= 0; --k) acc = fmaf(acc, t, c[k]);
+  return acc;
+}
+
+template <int N>
+__device__ inline float horner(const float (&c)[N], float t) {
+  float acc = c[N - 1];
+#pragma unroll
+  for (int k = N - 2; k >= 0; --k) acc = fmaf(acc, t, c[k]);
+  return acc;
+}
+
+// sin and cos of theta: k = rint(theta/pi), r = theta - k*pi in [-pi/2, pi/2]
+// (two-constant Cody-Waite), Taylor kernels to r^13 / r^14, sign (-1)^k.
+__device__ inline void sincos_fast(float theta, float* s, float* c) {
+  const float PI_HI = 3.14159274101257324219f;      // fp32(pi)
+  const float PI_LO = -8.74227800037248274685e-08f; // pi - PI_HI
+  const float INV_PI = 0.318309886183790671538f;
+  const float k = rintf(theta * INV_PI);
+  float r = fmaf(-k, PI_HI, theta);
+  r = fmaf(-k, PI_LO, r);
+  const float r2 = r * r;
+  float sp = horner(SIN_T, r2);
+  float sn = fmaf(r * r2, sp, r);
+  float cs = fmaf(r2, horner(COS_T, r2), 1.0f);
+  const uint32_t flip = ((uint32_t)(int)k & 1u) << 31;
+  *s = __uint_as_float(__float_as_uint(sn) ^ flip);
+  *c = __uint_as_float(__float_as_uint(cs) ^ flip);
+}
+
+// atan2(y, x) in (-pi, pi]: t = min/max in [0,1], atan(t) = t P(t^2), octant fix-up.
+__device__ inline float atan2_fast(float y, float x) {
+  const float ax = fabsf(x), ay = fabsf(y);
+  const float mx = fmaxf(ax, ay), mn = fminf(ax, ay);
+  const float t = mx > 0.0f ? mn * __builtin_amdgcn_rcpf(mx) : 0.0f;
+  float p = t * horner(ATAN_P, t * t);
+  p = ay > ax ? 1.57079632679489661923f - p : p;
+  p = x < 0.0f ? 3.14159265358979323846f - p : p;
+  return copysignf(p, y);
+}
+
+// B(y) = J1(x)/x and C(y) = J2(x)/x^2, x = sqrt(y).
+//   y <= 64: Chebyshev-fitted polynomials in t = y/32 - 1.
+//   y  > 64: Hankel asymptotics J_n ~ sqrt(2/(pi x)) (P_n cos chi_n - Q_n sin chi_n),
+//            chi_1 = x - 3pi/4, chi_2 = chi_1 - pi/2 (one shared sincos).
+__device__ inline void bessel_BC_poly(float y, float* B, float* C) {
+  const float t = fmaf(y, 0.03125f, -1.0f);
+  *B = horner(BESSEL_B, t);
+  *C = horner(BESSEL_C, t);
+}
+
+__device__ inline void bessel_BC_asym(float y, float* B, float* C) {
+  const float x = __builtin_amdgcn_sqrtf(y);
+  const float ix = __builtin_amdgcn_rcpf(x);
+  const float z = 0.125f * ix;
+  const float z2 = z * z;
+  const float P1 = horner(HANKEL_P1, z2), Q1 = z * horner(HANKEL_Q1, z2);
+  const float P2 = horner(HANKEL_P2, z2), Q2 = z * horner(HANKEL_Q2, z2);
+  const float amp = __builtin_amdgcn_sqrtf(0.636619772367581343f * ix);
+  float s, c;
+  sincos_fast(x - 2.35619449019234492885f, &s, &c);
+  const float j1 = amp * (P1 * c - Q1 * s);
+  const float j2 = amp * (P2 * s + Q2 * c);
+  *B = j1 * ix;
+  *C = j2 * ix * ix;
+}
+
+// Full-range evaluation for one lane (used where divergence is harmless).
+__device__ inline void bessel_BC(float y, float* B, float* C) {
+  if (y <= 64.0f) bessel_BC_poly(y, B, C);
+  else bessel_BC_asym(y, B, C);
+}
+
+// ---------------------------------------------------------------- wave64
+
+__device__ inline int lane_id() { return (int)(threadIdx.x & 63u); }
+
+__device__ inline float bcast(float v, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
+}
+__device__ inline int bcast(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+
+__device__ inline void wave_sync_lds() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+}  // namespace mhre

@@ -1,0 +1,205 @@
+"""Batched, device-resident MH-RE sessions (the engine's extended API).
+
+A ``Fit`` owns the reference-layout arrays of one problem (the same arrays
+``mcmcuda`` takes); a ``Session`` uploads any number of fits to the GPU once,
+runs iterations with the inputs resident in HBM, and copies results back into
+the fits' arrays.  A plain reference-style run is a session of one fit.
+
+Defaults follow the reference driver ``src/imgpu.py:259-424``: mid-prior
+``pcur`` (overwritten by the random start inside the prior), steps
+``(pmax - pmin) / 50``, ``tcur[:, s] = 0..nbeta-1``, ``n_cnt = 1`` and the
+geometric temperature ladder ``beta_i = betan ** (i / (nbeta - 1))``.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import (MhreFitSpec, MhreOptions, MhreStats, ptr, MHRE_MODEL_AUTO, MHRE_MODEL_SGRA9,
+                   MHRE_MODEL_SGRA13, MHRE_MODEL_USER, MHRE_MODEL_QUADSYS)
+
+MODELS = {"auto": MHRE_MODEL_AUTO, "sgra9": MHRE_MODEL_SGRA9, "sgra13": MHRE_MODEL_SGRA13,
+          "user": MHRE_MODEL_USER, "quadsys": MHRE_MODEL_QUADSYS}
+
+
+def geometric_beta(nbeta, beta1=1.0, betan=1e-4):
+    """Temperature ladder of src/imgpu.py:412-418 (float32)."""
+    beta = np.zeros(nbeta, np.float32)
+    if nbeta == 1:
+        beta[0] = beta1
+        return beta
+    bstp = (np.float32(betan) / np.float32(beta1)) ** (1.0 / np.float32(nbeta - 1))
+    for i in range(nbeta):
+        beta[i] = beta1 * bstp ** i
+    return beta
+
+
+def var_maps(pdescr):
+    """ivar / invar of src/imgpu.py:320-339."""
+    pdescr = np.asarray(pdescr, np.int32)
+    ivar = np.nonzero(pdescr)[0].astype(np.int32)
+    invar = np.full(pdescr.size, -1, np.int32)
+    invar[ivar] = np.arange(ivar.size, dtype=np.int32)
+    return ivar, invar
+
+
+class Fit:
+    """Arrays of one MH-RE problem in the reference layouts."""
+
+    def __init__(self, pdescr, pmint, pmaxt, coor, dat, std2r, nbeta=32, nseq=14, nburn=300, niter=500,
+                 model="auto", imodel=0, icoor=None, idat=None, ptotal=None, beta=None, beta1=1.0,
+                 betan=1e-4, nadj=100, npass=1, seed=0, adapt_target=0.30, pstp=None,
+                 init_chains=True, model_data=False, ndatm=None, it0=0):
+        f32, i32 = np.float32, np.int32
+        self.model = MODELS[model] if isinstance(model, str) else int(model)
+        self.imodel = int(imodel)
+        self.pdescr = np.ascontiguousarray(pdescr, i32)
+        self.nptot = self.pdescr.size
+        self.ivar, self.invar = var_maps(self.pdescr)
+        self.nprm = self.ivar.size
+        self.pmint = np.ascontiguousarray(pmint, f32)
+        self.pmaxt = np.ascontiguousarray(pmaxt, f32)
+        self.ptotal = np.zeros(self.nptot, f32) if ptotal is None else np.ascontiguousarray(ptotal, f32)
+        self.coor = np.ascontiguousarray(coor, f32)
+        self.dat = np.ascontiguousarray(dat, f32)
+        self.std2r = np.ascontiguousarray(std2r, f32)
+        self.icoor = np.zeros(1, i32) if icoor is None else np.ascontiguousarray(icoor, i32)
+        self.idat = np.zeros(1, i32) if idat is None else np.ascontiguousarray(idat, i32)
+        self.nbeta, self.nseq, self.nburn, self.niter = int(nbeta), int(nseq), int(nburn), int(niter)
+        self.nadj, self.npass = int(nadj), int(npass)
+        self.seed = int(np.uint64(seed))
+        self.adapt_target = float(adapt_target)
+        self.init_chains = bool(init_chains)
+        self.it0 = int(it0)
+        B, S, P, N, D = self.nbeta, self.nseq, self.nprm, self.nptot, self.dat.size
+        self.ndat = D
+        self.beta = geometric_beta(B, beta1, betan) if beta is None else np.ascontiguousarray(beta, f32)
+        pmin, pmax = self.pmint[self.ivar], self.pmaxt[self.ivar]
+        self.pcur = np.tile(self.ptotal, (B, S, 1)).astype(f32)
+        self.pcur[:, :, self.ivar] = (pmin + (pmax - pmin) / f32(2.0)).astype(f32)
+        if pstp is None:
+            step = ((pmax - pmin) / 50.0).astype(f32)
+        else:
+            step = np.broadcast_to(np.asarray(pstp, f32), (P,)).astype(f32)
+        self.pstp = np.ascontiguousarray(np.broadcast_to(step[:, None, None], (P, B, S)), f32)
+        self.tcur = np.ascontiguousarray(np.tile(np.arange(B, dtype=i32)[:, None], (1, S)))
+        self.n_cnt = np.ones((P, B, S), i32)
+        self.n_acpt = np.zeros((P, B, S), i32)
+        self.n_exch = np.zeros((B, S), i32)
+        self.n_hist = np.zeros((self.nadj, P, B, S), i32)
+        self.flag = np.zeros((B, S), i32)
+        self.chi2c = np.zeros((B, S), f32)
+        self.ptent = np.zeros((B, S), f32)
+        self.ptentn = np.zeros((B, S), i32)
+        self.pout = np.zeros((P, B, S, self.niter), f32)
+        self.tout = np.zeros((B, S, self.niter), i32)
+        self.chi2 = np.zeros((S, self.niter), f32)
+        self.rndst = np.zeros((B, S, 48), np.uint8)
+        if model_data:
+            nvis = int(self.idat[0]) if self.idat.size else 0
+            self.ndatm = int(ndatm) if ndatm is not None else B * S * (D + nvis)
+            self.datm = np.zeros(self.ndatm, f32)
+            self.chi2m = np.zeros((B, S, D), f32)
+        else:
+            self.ndatm = 0
+            self.datm = None
+            self.chi2m = None
+
+    def spec(self):
+        F, I = C.c_float, C.c_int
+        s = MhreFitSpec()
+        s.model, s.imodel = self.model, self.imodel
+        s.coor, s.dat, s.std2r = ptr(self.coor, F), ptr(self.dat, F), ptr(self.std2r, F)
+        s.icoor, s.idat = ptr(self.icoor, I), ptr(self.idat, I)
+        s.datm, s.chi2m = ptr(self.datm, F), ptr(self.chi2m, F)
+        s.pdescr, s.ivar, s.invar = ptr(self.pdescr, I), ptr(self.ivar, I), ptr(self.invar, I)
+        s.ptotal, s.pmint, s.pmaxt = ptr(self.ptotal, F), ptr(self.pmint, F), ptr(self.pmaxt, F)
+        s.pout, s.tout, s.chi2 = ptr(self.pout, F), ptr(self.tout, I), ptr(self.chi2, F)
+        s.chi2c, s.tcur, s.flag = ptr(self.chi2c, F), ptr(self.tcur, I), ptr(self.flag, I)
+        s.n_acpt, s.n_exch, s.n_cnt, s.n_hist = (ptr(self.n_acpt, I), ptr(self.n_exch, I),
+                                                 ptr(self.n_cnt, I), ptr(self.n_hist, I))
+        s.beta, s.pstp, s.ptent, s.ptentn = ptr(self.beta, F), ptr(self.pstp, F), ptr(self.ptent, F), ptr(self.ptentn, I)
+        s.pcur, s.rndst = ptr(self.pcur, F), ptr(self.rndst, C.c_ubyte)
+        s.seed = self.seed
+        s.ncoor, s.ndat, s.nicoor, s.nidat, s.ndatm = (self.coor.size, self.ndat, self.icoor.size,
+                                                       self.idat.size, self.ndatm)
+        s.nptot, s.nprm, s.nadj, s.npass = self.nptot, self.nprm, self.nadj, self.npass
+        s.nbeta, s.nseq, s.nburn, s.niter = self.nbeta, self.nseq, self.nburn, self.niter
+        s.adapt_target, s.init_chains, s.it0 = self.adapt_target, int(self.init_chains), self.it0
+        return s
+
+
+class Session:
+    """Device-resident batch of fits.
+
+    options: waves_per_block (1/2/4), persistent (0/1), graph_iters (>0 to
+    replay a captured cycle), trace_iters (single fit: record decisions),
+    force_generic (run imodel 1/2 through the compiled-in model.cuh).
+    """
+
+    def __init__(self, fits, lib=None, waves_per_block=0, persistent=-1, graph_iters=0, trace_iters=0,
+                 force_generic=False):
+        self.L = _lib.load(lib)
+        self.fits = list(fits)
+        self._specs = (MhreFitSpec * len(self.fits))(*[f.spec() for f in self.fits])
+        self.opt = MhreOptions()
+        self.opt.waves_per_block = int(waves_per_block)
+        self.opt.persistent = int(persistent)
+        self.opt.graph_iters = int(graph_iters)
+        self.opt.force_generic = int(bool(force_generic))
+        self.trace = None
+        if trace_iters:
+            if len(self.fits) != 1:
+                raise ValueError("decision traces need a single-fit session")
+            f = self.fits[0]
+            C_ = f.nbeta * f.nseq
+            self.trace = dict(dec=np.full((trace_iters, f.nprm, C_), -1, np.int32),
+                              chi2=np.zeros((trace_iters, f.nprm, C_), np.float32),
+                              ex=np.full((trace_iters, f.nbeta, f.nseq), -1, np.int32))
+            self.opt.trace_iters = int(trace_iters)
+            self.opt.trace_dec = ptr(self.trace["dec"], C.c_int)
+            self.opt.trace_chi2 = ptr(self.trace["chi2"], C.c_float)
+            self.opt.trace_ex = ptr(self.trace["ex"], C.c_int)
+        self.h = C.c_void_p()
+        _lib.check(self.L, self.L.mhre_session_create(self._specs, len(self.fits), C.byref(self.opt),
+                                                      C.byref(self.h)), "mhre_session_create")
+
+    def run(self, niter):
+        _lib.check(self.L, self.L.mhre_session_run(self.h, int(niter)), "mhre_session_run")
+
+    def sync(self):
+        _lib.check(self.L, self.L.mhre_session_sync(self.h), "mhre_session_sync")
+
+    def download(self, model_data=False):
+        _lib.check(self.L, self.L.mhre_session_download(self.h, int(bool(model_data))), "mhre_session_download")
+
+    def stats(self):
+        st = MhreStats()
+        _lib.check(self.L, self.L.mhre_session_stats(self.h, C.byref(st)), "mhre_session_stats")
+        return {k: getattr(st, k) for k, _ in MhreStats._fields_}
+
+    def close(self):
+        if self.h:
+            self.L.mhre_session_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def run_fits(fits, **options):
+    """Run every fit to completion (nburn + niter) in one session and download."""
+    with Session(fits, **options) as s:
+        total = max(f.nburn + f.niter - f.it0 for f in fits)
+        s.run(total)
+        s.download(model_data=any(f.datm is not None for f in fits))
+        return s.stats()

@@ -1,0 +1,169 @@
+/*
+ * mhre.h -- C ABI of the MI355X MH-RE (parallel-tempering MCMC) engine.
+ *
+ * Plain pointers and sizes only.  All functions return 0 on success and a
+ * negative MHRE_E* code on failure; mhre_last_error() gives the message.
+ * (The reference's functions return void and call exit(1) on CUDA errors,
+ * src/mcmcjob.cu:212-219.)
+ *
+ * Part 1 -- drop-in replacements for the reference's C ABI (src/mcmc.h) with
+ * the same argument order and meanings.  Host buffers are C-contiguous arrays
+ * in the reference layouts (src/mcmc_interf.pyx:232-263); results are written
+ * back in place.
+ *
+ * Part 2 -- extensions: device-resident sessions over batches of fits (each
+ * fit described by the same arrays), statistics and decision traces.
+ */
+#ifndef MHRE_H
+#define MHRE_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MHRE_OK 0
+#define MHRE_EINVAL (-1)     /* inconsistent sizes / arguments */
+#define MHRE_EHIP (-2)       /* HIP runtime error */
+#define MHRE_EMODEL (-3)     /* model() returned an error code (e.g. bad imodel) */
+#define MHRE_ENOMEM (-4)     /* device allocation failed */
+
+/* ------------------------------------------------------------ Part 1 */
+
+/* MH-RE fit: burn-in with step adaptation, then sampling.
+ * Replaces mcmc_cuda of src/mcmc.h:7-20 (src/mcmcjob.cu:475-545, run_mcmc of
+ * src/gpu_mcmc.cu:682-763).  Arrays: coor[ncoor] dat[ndat] std2r[ndat]
+ * icoor[nicoor] idat[nidat] datm[ndatm] chi2m[nbeta][nseq][ndat]
+ * pdescr/invar/ptotal/pmint/pmaxt[nptot] ivar[nprm]
+ * pout[nprm][nbeta][nseq][niter] tout[nbeta][nseq][niter] chi2[nseq][niter]
+ * chi2c/tcur/flag/ptent/ptentn[nbeta][nseq] n_acpt/n_cnt/pstp[nprm][nbeta][nseq]
+ * n_exch[nbeta][nseq] n_hist[nadj][nprm][nbeta][nseq] beta[nbeta]
+ * pcur[nbeta][nseq][nptot] rndst[nbeta][nseq][48].
+ * rndst receives each chain's resume record (seed, next iteration, chain id)
+ * instead of cuRAND states.  datm/chi2m receive the model data and chi^2
+ * terms of the final state of every chain. */
+int mcmc_cuda(float *coor, float *dat, float *std2r,
+              int *icoor, int *idat, float *datm, float *chi2m,
+              int *pdescr, int *ivar, int *invar, float *ptotal,
+              float *pmint, float *pmaxt,
+              float *pout, int *tout, float *chi2,
+              float *chi2c, int *tcur, int *flag,
+              int *n_acpt, int *n_exch, int *n_cnt, int *n_hist,
+              float *beta, float *pstp, float *ptent, int *ptentn,
+              float *pcur, unsigned char *rndst,
+              unsigned long long seed, int imodel,
+              int ncoor, int ndat, int nicoor, int nidat, int ndatm,
+              int nptot, int nprm, int nadj, int npass,
+              int nbeta, int nseq, int nburn, int niter);
+
+/* Batched model data and chi^2 for every [nbeta][nseq] parameter vector.
+ * Replaces calc_modchi2 of src/mcmc.h:23-27 (src/mcmcjob.cu:227-254,
+ * run_calcmodchi2 of src/gpu_mcmc.cu:769-783).  Chains with flag == 0 are
+ * not evaluated; chi2c is the sum of each chain's chi2m row. */
+int calc_modchi2(float *coor, float *dat, float *std2r,
+                 int *icoor, int *idat, float *datm, float *chi2m,
+                 float *pcur, float *chi2c, int *flag, int imodel,
+                 int ncoor, int ndat, int nicoor, int nidat, int ndatm,
+                 int nptot, int nbeta, int nseq);
+
+/* Model data only (pass 0) for every parameter vector.
+ * Replaces calc_model of src/mcmc.h:30-32 (src/mcmcjob.cu:257-298,
+ * run_calcmodel of src/gpu_mcmc.cu:790-802).  coor holds interleaved
+ * (u, v) pairs, the layout model() reads. */
+int calc_model(float *coor, int *idat, float *datm, float *pcur,
+               int imodel, int ncoor, int ndat, int nidat,
+               int ndatm, int nptot, int nbeta, int nseq);
+
+/* Release every device resource of this process on the current device.
+ * Replaces reset_gpu / cudaDeviceReset of src/mcmc_interf.pyx:342-343. */
+int reset_gpu(void);
+
+/* Text listing of the visible GPUs (name, gfx target, CUs, memory), one line
+ * each.  Replaces the list_gpus program of src/list_gpus.cu:13-22. */
+int list_gpus(char *buf, int buflen);
+
+/* ------------------------------------------------------------ Part 2 */
+
+#define MHRE_MODEL_AUTO 0      /* from imodel like mcmc_cuda */
+#define MHRE_MODEL_SGRA9 1     /* native 9-parameter ring + Gaussian */
+#define MHRE_MODEL_SGRA13 2    /* native 13-parameter ring + Gaussian */
+#define MHRE_MODEL_USER 10     /* the library's compiled-in model.cuh */
+#define MHRE_MODEL_QUADSYS 20  /* sum_e (dat_e - x'Q_e x - b_e.x)^2 std2r_e;
+                                  coor = [Q (ndat*nptot*nptot), b (ndat*nptot)] */
+
+/* One fit: the arrays and sizes of mcmc_cuda plus options.  No reference
+ * counterpart: the batched engine's unit of work. */
+typedef struct MhreFitSpec {
+  int model;                    /* MHRE_MODEL_* */
+  int imodel;
+  float *coor, *dat, *std2r;
+  int *icoor, *idat;
+  float *datm, *chi2m;          /* may be NULL: final-state model data not wanted */
+  int *pdescr, *ivar, *invar;
+  float *ptotal, *pmint, *pmaxt;
+  float *pout; int *tout; float *chi2;
+  float *chi2c; int *tcur; int *flag;
+  int *n_acpt, *n_exch, *n_cnt, *n_hist;
+  float *beta, *pstp, *ptent; int *ptentn;
+  float *pcur; unsigned char *rndst;   /* rndst may be NULL */
+  unsigned long long seed;
+  int ncoor, ndat, nicoor, nidat, ndatm;
+  int nptot, nprm, nadj, npass, nbeta, nseq, nburn, niter;
+  float adapt_target;           /* step-adaptation threshold; 0 -> 0.30 */
+  int init_chains;              /* 1: random start in the prior + initial chi^2
+                                   (reference); 0: start from pcur and chi2c */
+  int it0;                      /* first iteration index (resume) */
+} MhreFitSpec;
+
+typedef struct MhreOptions {
+  int waves_per_block;          /* native step kernel: 1, 2 or 4; 0 = default */
+  int persistent;               /* 1: persistent step kernel with a device work
+                                   queue; 0: one wave per chain; -1 = default */
+  int graph_iters;              /* >0: capture this many iterations in a
+                                   hipGraph and replay it */
+  int trace_iters;              /* single-fit sessions: record the decisions of
+                                   iterations it0 .. it0+trace_iters-1 */
+  int *trace_dec;               /* [trace_iters][nprm][nbeta*nseq]:
+                                   k | inprior<<8 | accept<<9 */
+  float *trace_chi2;            /* same shape: proposal chi^2 (0 if out of prior) */
+  int *trace_ex;                /* [trace_iters][nbeta][nseq]: k | accept<<16 */
+  int force_generic;            /* run imodel 1/2 through the compiled-in model.cuh */
+} MhreOptions;
+
+typedef struct MhreStats {
+  double run_ms;                /* device time of the last mhre_session_run */
+  unsigned long long chi2_evals;/* chi^2 evaluations performed (in-prior proposals) */
+  unsigned long long proposals; /* proposals made */
+  double vis_evals;             /* visibility model evaluations performed */
+  double cph_evals;             /* closure-phase terms evaluated */
+  int iters_done;               /* iterations run so far */
+  int nfits;
+} MhreStats;
+
+/* Upload a batch of fits and initialise their chains (per init_chains). */
+int mhre_session_create(const MhreFitSpec *fits, int nfits, const MhreOptions *opt, void **session);
+/* Run niter MH-RE iterations of every fit (asynchronous to the host). */
+int mhre_session_run(void *session, int niter);
+/* Wait for the session's stream. */
+int mhre_session_sync(void *session);
+/* Copy chain state and outputs back into the fit specs' host arrays; with
+ * model_data != 0 also fill datm/chi2m for the final state. */
+int mhre_session_download(void *session, int model_data);
+int mhre_session_stats(void *session, MhreStats *stats);
+int mhre_session_destroy(void *session);
+/* create + run(nburn + niter) + download + destroy. */
+int mhre_run_batch(const MhreFitSpec *fits, int nfits, const MhreOptions *opt, MhreStats *stats);
+
+const char *mhre_last_error(void);
+int mhre_version(void);
+/* Name of the model.cuh compiled into this library build. */
+const char *mhre_user_model(void);
+/* 1 if imodel 1/2 use the native fused kernels in this build. */
+int mhre_native_eht(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MHRE_H */

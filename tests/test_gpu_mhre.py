@@ -1,0 +1,136 @@
+"""MH-RE decision parity: device engine vs the float64 oracle, in lock step.
+
+Every iteration the oracle restarts from the state the GPU downloaded, runs
+the same iteration (same Philox counters) and the two are compared:
+
+* random coordinate picks and exchange pair picks: identical everywhere;
+* prior checks, accept and swap decisions: identical except where the
+  oracle's decision margin |log u - x| is within the chi^2 tolerance band
+  (tests.helpers.tie_tol: 1e-5 relative chi^2 -> 0.5 beta 1e-5 (chi2 + chi2c));
+* chain state of every sequence without a near-tie: counters, histories,
+  temperature labels and step sizes bit-identical, parameters identical,
+  chi^2 within 1e-5 relative.
+"""
+import numpy as np
+import pytest
+
+from cuda_mcmc_amd import synthdata
+from cuda_mcmc_amd.engine import Session
+from oracle import mhre
+from tests.helpers import eht_fit, oracle_eht_chi2, tie_tol
+
+pytestmark = pytest.mark.gpu
+
+STATE = ("pcur", "chi2c", "pstp", "n_cnt", "n_acpt", "n_hist", "n_exch", "tcur", "flag", "ptent", "ptentn")
+
+
+def _problem(fit, chi2_fn):
+    return mhre.Problem(chi2_fn, fit.pdescr, fit.pmint, fit.pmaxt, fit.beta, fit.nseq, ptotal=fit.ptotal,
+                        nadj=fit.nadj, seed=fit.seed, adapt_target=fit.adapt_target)
+
+
+def _state_from(pb, fit):
+    st = mhre.State(pb, niter=fit.niter)
+    for k in STATE + ("pout", "tout", "chi2"):
+        setattr(st, k, getattr(fit, k).copy())
+    return st
+
+
+def _compare_iteration(pb, fit, st, tr_mh, tr_ex, gpu_dec, gpu_ex, it):
+    B, S, P = fit.nbeta, fit.nseq, fit.nprm
+    C = B * S
+    bad_chain = np.zeros(C, bool)
+    ntie = 0
+    for s in range(P):
+        o = tr_mh[s]
+        g = gpu_dec[s]
+        gk = g & 0xFF
+        gin = (g >> 8) & 1
+        gacc = (g >> 9) & 1
+        live = ~bad_chain
+        assert np.array_equal(gk[live], o["k"][live]), "coordinate pick differs (iteration %d, substep %d)" % (it, s)
+        assert np.array_equal(gin[live].astype(bool), o["inprior"][live]), "prior check differs"
+        beta = fit.beta[np.arange(C) // S]
+        tie = o["inprior"] & (np.abs(o["margin"]) < tie_tol(beta, o["chi2"], o["chi2"]))
+        ntie += int(np.sum(tie & live))
+        diff = live & (gacc.astype(bool) != o["accept"])
+        assert not np.any(diff & ~tie), "accept decision differs away from a tie (iteration %d, substep %d)" % (it, s)
+        bad_chain |= tie
+    bad_seq = bad_chain.reshape(B, S).any(axis=0)
+    o = tr_ex[0]
+    if B > 1:
+        for a in range(B):
+            gk = gpu_ex[a] & 0xFFFF
+            gacc = (gpu_ex[a] >> 16) & 1
+            ok = ~bad_seq
+            assert np.array_equal(gk[ok], o["k"][a][ok]), "exchange pair pick differs"
+            kk = o["k"][a]
+            db = np.abs(fit.beta[kk + 1].astype(np.float64) - fit.beta[kk])
+            c0 = np.abs(st.chi2c[kk, np.arange(S)])
+            tie = np.abs(o["margin"][a]) < tie_tol(db, c0, c0)
+            d = ok & (gacc.astype(bool) != o["accept"][a])
+            assert not np.any(d & ~tie), "swap decision differs away from a tie"
+            bad_seq |= tie
+    return bad_seq, ntie
+
+
+def _lockstep(fit, chi2_fn, niters):
+    pb = _problem(fit, chi2_fn)
+    total_ties = 0
+    compared = 0
+    with Session([fit], trace_iters=niters) as ses:
+        ses.download()
+        for it in range(niters):
+            st = _state_from(pb, fit)
+            trace = {}
+            burn = it < fit.nburn
+            mhre.iteration(pb, st, it, burn, itr_out=None if burn else it - fit.nburn, trace=trace)
+            ses.run(1)
+            ses.download()
+            dec = ses.trace["dec"][it]
+            ex = ses.trace["ex"][it]
+            bad_seq, nt = _compare_iteration(pb, fit, st, trace["mh"], trace["ex"], dec, ex, it)
+            total_ties += nt
+            ok = ~bad_seq
+            compared += int(ok.sum())
+            B, S = fit.nbeta, fit.nseq
+            for name in ("n_cnt", "n_acpt", "n_hist", "tcur", "n_exch", "flag", "ptentn"):
+                g = getattr(fit, name)
+                o = getattr(st, name)
+                assert np.array_equal(g[..., ok], o[..., ok]), "%s differs at iteration %d" % (name, it)
+            np.testing.assert_array_equal(fit.pstp[..., ok], st.pstp[..., ok])
+            np.testing.assert_allclose(fit.pcur[:, ok], st.pcur[:, ok], rtol=1e-6, atol=0)
+            np.testing.assert_allclose(fit.chi2c[:, ok], st.chi2c[:, ok], rtol=1e-5)
+            if not burn:
+                itr = it - fit.nburn
+                np.testing.assert_allclose(fit.pout[:, :, ok, itr], st.pout[:, :, ok, itr], rtol=1e-6)
+                np.testing.assert_array_equal(fit.tout[:, ok, itr], st.tout[:, ok, itr])
+    return compared, total_ties
+
+
+def test_init_bit_exact():
+    d = synthdata.ring_gauss_dataset(nvis=400, ncph=120, seed=21)
+    fit = eht_fit(d, nbeta=8, nseq=8, nburn=0, niter=0, seed=12345)
+    with Session([fit]) as ses:
+        ses.download()
+    pb = _problem(fit, oracle_eht_chi2(d))
+    st = mhre.State(pb, niter=0)
+    mhre.init_pcur(pb, st)
+    np.testing.assert_array_equal(fit.pcur, st.pcur)
+    np.testing.assert_allclose(fit.chi2c, st.chi2c, rtol=1e-5)
+    assert np.all(fit.flag == 1)
+
+
+def test_lockstep_eht_burnin_and_sampling():
+    d = synthdata.ring_gauss_dataset(nvis=500, ncph=150, seed=22)
+    fit = eht_fit(d, nbeta=8, nseq=16, nburn=3, niter=3, seed=77)
+    compared, ties = _lockstep(fit, oracle_eht_chi2(d), 6)
+    assert compared >= 6 * 16 * 0.5
+
+
+def test_lockstep_adaptation_threshold_and_short_history():
+    # nadj smaller than the run so the history ring wraps; adapt target 0.25
+    d = synthdata.ring_gauss_dataset(nvis=300, ncph=90, seed=23)
+    fit = eht_fit(d, nbeta=4, nseq=16, nburn=8, niter=1, seed=5, nadj=3, adapt_target=0.25)
+    compared, _ = _lockstep(fit, oracle_eht_chi2(d), 9)
+    assert compared >= 9 * 16 * 0.5
