@@ -1,0 +1,238 @@
+#!/usr/bin/env python3
+"""Benchmark of the MH-RE engine on one MI355X.
+
+Headline workload (one "step" = one MH-RE iteration: nprm = 9 single-parameter
+proposals for every chain followed by the replica exchange):
+    plain MH-RE fit of the 9-parameter ring+Gaussian visibility model,
+    32 temperatures x 128 sequences (4096 chains), 2000 visibility amplitudes
+    + 600 closure phases (8 stations, 6 h tracks at 230 GHz), 5000 burn-in +
+    20000 sampling iterations.  --warmup + --steps iterations are run
+    (default 100 + 24900 = the whole fit); only --steps are timed.
+value = visibility model evaluations performed per second (G/s), inputs
+resident in HBM, whole job (one GPU).
+
+Also reported:
+  roofline      the fused step kernel's algorithmic FP32 rate (170 flop per
+                visibility-eval, 15 per closure term, 40 per chain setup)
+                over its HIP-event duration, against the 157.3 TFLOP/s peak;
+  cpu_baseline  the float64 C/OpenMP oracle port on the host cores;
+  batched       64 uneven fits in one session vs the same fits one session
+                at a time.
+
+Prints exactly one JSON line on stdout; progress goes to stderr.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "G visibility-evals/s in MH-RE fit (batched vs plain), % of FP32 VALU peak"
+PEAK_FP32_TFLOPS = 157.3
+F_VIS, F_CPH, F_CHAIN = 170.0, 15.0, 40.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def eht_fit(d, nbeta, nseq, nburn, niter, seed):
+    from cuda_mcmc_amd import synthdata
+    from cuda_mcmc_amd.engine import Fit
+    idat = np.array([d["nvis"], nbeta * nseq * (d["nvis"] + d["ncph"]), 0], np.int32)
+    return Fit(synthdata.EHT_PDESCR, synthdata.EHT_PMIN, synthdata.EHT_PMAX, d["coor"], d["dat"], d["std2r"],
+               nbeta=nbeta, nseq=nseq, nburn=nburn, niter=niter, model="sgra9", imodel=1,
+               icoor=d["icoor"], idat=idat, npass=2, seed=seed)
+
+
+def traffic_from_profiles():
+    """HBM bytes per step launch from the committed PMC summary, if present."""
+    p = os.path.join(ROOT, "profiles", "sgra9_step_config2_pmc.txt")
+    if not os.path.isfile(p):
+        return None
+    for line in open(p):
+        if line.startswith("HBM_BYTES_PER_LAUNCH"):
+            try:
+                return float(line.split()[1])
+            except (IndexError, ValueError):
+                return None
+    return None
+
+
+def cpu_baseline(d, nthreads):
+    """Time the float64 C/OpenMP oracle port on 2 iterations of the same fit."""
+    from oracle import cport
+    cport.load()
+    fit = eht_fit(d, 32, 128, nburn=5000, niter=0, seed=0)
+    # start from a state drawn like the device's (uniform in the prior)
+    rng = np.random.default_rng(0)
+    pmin, pmax = fit.pmint.astype(np.float64), fit.pmaxt.astype(np.float64)
+    fit.pcur[:] = (pmin + (pmax - pmin) * rng.uniform(size=fit.pcur.shape)).astype(np.float32)
+    fit.chi2c[:] = cport.eht_chi2_batch(fit.pcur.reshape(-1, 9), d, nthreads).reshape(fit.chi2c.shape)
+    t = time.time()
+    nev = 0
+    iters = 2
+    for it in range(iters):
+        nev += cport.eht_iteration(fit, d, it, nthreads)
+    dt = time.time() - t
+    return dict(value=nev * d["nvis"] / dt / 1e9, unit="G visibility-evals/s", cores=int(nthreads),
+                kind="port",
+                sample="%d MH-RE iterations (all 4096 chains, %d chi^2 evaluations of 2000 vis + 600 "
+                       "closures) of the same fit, float64 C/OpenMP port of the oracle, %.1f s" % (iters, nev, dt))
+
+
+def batched_vs_plain(iters, nfits, log_every=16):
+    """Config-3 style batch: uneven fits in one session vs one session per fit."""
+    from cuda_mcmc_amd import synthdata
+    from cuda_mcmc_amd.engine import Session
+    fits, datas = [], []
+    for f in range(nfits):
+        rng = np.random.default_rng(300 + f)
+        nsta = int(rng.integers(5, 11))
+        nvis = int(np.exp(rng.uniform(np.log(500), np.log(20000))))
+        per_time = nsta * (nsta - 1) // 2
+        ntri = nsta * (nsta - 1) * (nsta - 2) // 6
+        ncph = min(int(0.3 * nvis), int(nvis / per_time * ntri * 0.8))
+        d = synthdata.ring_gauss_dataset(nvis=nvis, ncph=ncph, nsta=nsta, seed=300 + f)
+        nb = int(rng.integers(16, 49))
+        ns = int(rng.integers(64, 257))
+        fits.append((d, nb, ns))
+        if (f + 1) % log_every == 0:
+            log("  generated %d/%d fits" % (f + 1, nfits))
+    mk = lambda d, nb, ns, f: eht_fit(d, nb, ns, nburn=iters + 2, niter=0, seed=300 + f)
+    total_vis = 0.0
+    with Session([mk(d, nb, ns, f) for f, (d, nb, ns) in enumerate(fits)], persistent=1) as s:
+        s.run(1)
+        s.sync()
+        e0 = s.stats()
+        t0 = time.time()
+        s.run(iters)
+        s.sync()
+        tb = time.time() - t0
+        e1 = s.stats()
+        total_vis = e1["vis_evals"] - e0["vis_evals"]
+    batched = total_vis / tb
+    tp = 0.0
+    plain_vis = 0.0
+    for f, (d, nb, ns) in enumerate(fits):
+        with Session([mk(d, nb, ns, f)]) as s:
+            s.run(1)
+            s.sync()
+            e0 = s.stats()
+            t0 = time.time()
+            s.run(iters)
+            s.sync()
+            tp += time.time() - t0
+            plain_vis += s.stats()["vis_evals"] - e0["vis_evals"]
+    plain = plain_vis / tp
+    return dict(fits=nfits, iterations=iters,
+                chains=int(sum(nb * ns for _, nb, ns in fits)),
+                vis_total=int(sum(d["nvis"] for d, _, _ in fits)),
+                batched_gvis_per_s=batched / 1e9, plain_sequential_gvis_per_s=plain / 1e9,
+                ratio=batched / plain,
+                pct_fp32_peak_batched=100.0 * batched * F_VIS / (PEAK_FP32_TFLOPS * 1e12))
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=24900)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-fits", type=int, default=64)
+    ap.add_argument("--batched-iters", type=int, default=4)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py runs on one GPU only: every fit and sequence is independent, so the job "
+              "shards by fit or sequence with no exchange during sampling (see the design notes); "
+              "no multi-GPU figure is produced.", file=sys.stderr)
+        return 2
+
+    from cuda_mcmc_amd import synthdata, mcmc_interf
+    from cuda_mcmc_amd.engine import Session
+
+    log(mcmc_interf.list_gpus().strip())
+    d = synthdata.ring_gauss_dataset(nvis=2000, ncph=600, nsta=8, seed=2)
+    nburn_cfg, niter_cfg = 5000, 20000
+    total = args.warmup + args.steps
+    nburn = min(nburn_cfg, total)
+    niter = max(0, min(niter_cfg, total - nburn))
+    fit = eht_fit(d, 32, 128, nburn=nburn, niter=niter, seed=0)
+    C = 32 * 128
+    per_eval = d["nvis"] * F_VIS + d["ncph"] * F_CPH + F_CHAIN
+    with Session([fit]) as ses:
+        # warmup: the first iterations, the last ones with per-launch HIP events
+        ntime = min(args.warmup, 50)
+        if args.warmup - ntime > 0:
+            ses.run(args.warmup - ntime)
+        ses.sync()
+        e0 = ses.stats()
+        step_ms, exch_ms = ses.time_kernels(ntime) if ntime > 0 else (float("nan"), float("nan"))
+        e1 = ses.stats()
+        evals_per_launch = (e1["chi2_evals"] - e0["chi2_evals"]) / max(ntime, 1)
+        log("warmup done: step kernel %.4f ms, exchange %.4f ms per iteration" % (step_ms, exch_ms))
+        # timed region
+        ses.sync()
+        t0 = time.time()
+        ses.run(args.steps)
+        ses.sync()
+        wall = time.time() - t0
+        e2 = ses.stats()
+        gpu_ms = e2["run_ms"]
+        log("timed %d iterations: %.3f s wall, %.3f s device" % (args.steps, wall, gpu_ms / 1e3))
+        vis = e2["vis_evals"] - e1["vis_evals"]
+        cph = e2["cph_evals"] - e1["cph_evals"]
+        ev = e2["chi2_evals"] - e1["chi2_evals"]
+        props = e2["proposals"] - e1["proposals"]
+    value = vis / wall / 1e9
+    achieved = evals_per_launch * per_eval / (step_ms * 1e-3) / 1e12
+    flop_rate = (vis * F_VIS + cph * F_CPH + ev * F_CHAIN) / wall / 1e12
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "G visibility-evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / max(args.steps, 1), 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic: ring+Gaussian source, 8 stations, 2000 amplitudes + 600 closure phases, data seed 2",
+        "config": {"workload": "plain MH-RE fit, 9-param ring+Gaussian, 32 temperatures x 128 sequences, "
+                               "2000 vis + 600 closures, 5000 burn-in + 20000 sampling iterations",
+                   "nbeta": 32, "nseq": 128, "nvis": 2000, "ncph": 600, "nprm": 9,
+                   "nburn": nburn_cfg, "niter": niter_cfg, "philox_seed": 0},
+        "pct_fp32_peak": round(100.0 * flop_rate / PEAK_FP32_TFLOPS, 2),
+        "proposals_per_s": props / wall,
+        "chi2_evals_per_s": ev / wall,
+        "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+                     "traffic": traffic_from_profiles(),
+                     "kernel": "k_step_eht<Sgra9>", "launch_ms": round(step_ms, 5),
+                     "exchange_ms": round(exch_ms, 5),
+                     "flop_per_launch": evals_per_launch * per_eval},
+    }
+    if not args.no_batched:
+        log("batched vs plain (%d fits)..." % args.batched_fits)
+        out["batched"] = batched_vs_plain(args.batched_iters, args.batched_fits)
+    if not args.no_cpu_baseline:
+        nth = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        log("cpu baseline on %d threads..." % nth)
+        out["cpu_baseline"] = cpu_baseline(d, nth)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -120,6 +120,8 @@ def _declare(L):
     L.mhre_session_create.argtypes = [C.POINTER(MhreFitSpec), C.c_int, C.POINTER(MhreOptions), C.POINTER(vp)]
     L.mhre_session_run.restype = C.c_int
     L.mhre_session_run.argtypes = [vp, C.c_int]
+    L.mhre_session_time_kernels.restype = C.c_int
+    L.mhre_session_time_kernels.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
     L.mhre_session_sync.restype = C.c_int
     L.mhre_session_sync.argtypes = [vp]
     L.mhre_session_download.restype = C.c_int

@@ -391,6 +391,45 @@ int launch_iteration(Session* S) {
   return MHRE_OK;
 }
 
+// Iterations with HIP events around every launch (on the session stream):
+// average device time per launch of the step kernels and of the exchange.
+int time_kernels(Session* S, int niter, double* step_ms, double* exch_ms) {
+  (void)hipGetLastError();
+  hipEvent_t e[3];
+  for (int i = 0; i < 3; ++i) HIPCHK(hipEventCreate(&e[i]));
+  double st = 0.0, ex = 0.0;
+  int rc = MHRE_OK;
+  for (int it = 0; it < niter && rc == MHRE_OK; ++it) {
+    HIPCHK(hipEventRecord(e[0], S->stream));
+    for (auto& k : S->kinds) {
+      const StepLaunch L = make_launch(S, k);
+      hipError_t err = hipSuccess;
+      if (k.kind == K_SGRA9) err = launch_step_eht(MODEL_SGRA9, L, S->stream);
+      else if (k.kind == K_SGRA13) err = launch_step_eht(MODEL_SGRA13, L, S->stream);
+      else if (k.kind == K_QUAD) err = launch_step_quad(k.quad_n, L, S->stream);
+      else err = launch_step_user(L, S->stream);
+      HIPCHK(err);
+    }
+    HIPCHK(hipEventRecord(e[1], S->stream));
+    StepLaunch L{};
+    L.fits = S->d_fits;
+    L.clk = S->d_clk;
+    HIPCHK(launch_exchange(L, S->d_x, S->n_x, S->stream));
+    HIPCHK(hipEventRecord(e[2], S->stream));
+    HIPCHK(hipEventSynchronize(e[2]));
+    float a = 0.0f, b = 0.0f;
+    HIPCHK(hipEventElapsedTime(&a, e[0], e[1]));
+    HIPCHK(hipEventElapsedTime(&b, e[1], e[2]));
+    st += a;
+    ex += b;
+  }
+  for (int i = 0; i < 3; ++i) hipEventDestroy(e[i]);
+  S->iters_done += niter;
+  if (step_ms) *step_ms = niter > 0 ? st / niter : 0.0;
+  if (exch_ms) *exch_ms = niter > 0 ? ex / niter : 0.0;
+  return rc;
+}
+
 int launch_eval_all(Session* S, int outputs, int respect_flag) {
   for (auto& k : S->kinds) {
     const int n = (int)k.eval.size();
@@ -842,6 +881,11 @@ int mhre_session_create(const MhreFitSpec* fits, int nfits, const MhreOptions* o
 int mhre_session_run(void* session, int niter) {
   if (!session) FAIL(MHRE_EINVAL, "NULL session");
   return run(static_cast<Session*>(session), niter);
+}
+
+int mhre_session_time_kernels(void* session, int niter, double* step_ms, double* exch_ms) {
+  if (!session) FAIL(MHRE_EINVAL, "NULL session");
+  return time_kernels(static_cast<Session*>(session), niter, step_ms, exch_ms);
 }
 
 int mhre_session_sync(void* session) {

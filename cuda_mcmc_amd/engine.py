@@ -167,6 +167,13 @@ class Session:
     def run(self, niter):
         _lib.check(self.L, self.L.mhre_session_run(self.h, int(niter)), "mhre_session_run")
 
+    def time_kernels(self, niter):
+        """Run niter iterations with per-launch HIP events; (step_ms, exchange_ms) per iteration."""
+        a, b = C.c_double(), C.c_double()
+        _lib.check(self.L, self.L.mhre_session_time_kernels(self.h, int(niter), C.byref(a), C.byref(b)),
+                   "mhre_session_time_kernels")
+        return a.value, b.value
+
     def sync(self):
         _lib.check(self.L, self.L.mhre_session_sync(self.h), "mhre_session_sync")
 

@@ -94,8 +94,10 @@ def ring_gauss_dataset(nvis=2000, ncph=600, nsta=8, seed=2, truth=None,
     while int(hours * 3600 / dt_s) * per_time * 0.5 < nvis:
         dt_s /= 2.0
     recs = _station_tracks(rng, nsta, dec, hours, dt_s, max_glam, freq_hz)
-    times = np.array(sorted(set(r[0] for r in recs)))
-    bytime = {t: [r for r in recs if r[0] == t] for t in times}
+    bytime = {}
+    for r in recs:
+        bytime.setdefault(r[0], []).append(r)
+    times = np.array(sorted(bytime))
     order = rng.permutation(times)
     chosen, count = [], 0
     for t in order:

@@ -145,6 +145,10 @@ typedef struct MhreStats {
 int mhre_session_create(const MhreFitSpec *fits, int nfits, const MhreOptions *opt, void **session);
 /* Run niter MH-RE iterations of every fit (asynchronous to the host). */
 int mhre_session_run(void *session, int niter);
+/* Run niter iterations timing every launch with HIP events on the session's
+ * stream; returns the average device time per iteration of the step
+ * kernel(s) and of the exchange kernel, in milliseconds. */
+int mhre_session_time_kernels(void *session, int niter, double *step_ms, double *exch_ms);
 /* Wait for the session's stream. */
 int mhre_session_sync(void *session);
 /* Copy chain state and outputs back into the fit specs' host arrays; with
