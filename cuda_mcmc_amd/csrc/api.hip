@@ -26,6 +26,7 @@
 #include "fitdesc.h"
 #include "engine.h"
 #include "mhre.h"
+#include "mhre_device.h"
 
 using namespace mhre;
 
@@ -463,9 +464,6 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
   if (opt) S->opt = *opt;
   S->wpb = S->opt.waves_per_block == 1 || S->opt.waves_per_block == 2 ? S->opt.waves_per_block : 4;
   S->persistent = S->opt.persistent > 0 ? 1 : 0;
-  HIPCHK(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
-  HIPCHK(hipEventCreate(&S->ev0));
-  HIPCHK(hipEventCreate(&S->ev1));
 
   const int it0 = specs[0].it0;
   S->fits.resize(nfits);
@@ -520,6 +518,10 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
     h.o_mc = A.reserve<CCalcModel>(1);
     h.d.ndatm = (long long)ndatm;
   }
+  // all arguments are validated and packed: now the device resources
+  HIPCHK(hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking));
+  HIPCHK(hipEventCreate(&S->ev0));
+  HIPCHK(hipEventCreate(&S->ev1));
   S->arena_bytes = A.bytes;
   HIPCHK(hipMalloc((void**)&S->arena, A.bytes));
   CHK(dev_alloc(S, &S->d_fits, nfits));
@@ -866,6 +868,11 @@ MhreFitSpec legacy_spec(float* coor, float* dat, float* std2r, int* icoor, int* 
 extern "C" {
 
 const char* mhre_last_error(void) { return g_err.c_str(); }
+
+void mhre_philox4x32_10(const unsigned int ctr[4], const unsigned int key[2], unsigned int out[4]) {
+  const U4 r = philox4x32_10(ctr[0], ctr[1], ctr[2], ctr[3], key[0], key[1]);
+  out[0] = r.x; out[1] = r.y; out[2] = r.z; out[3] = r.w;
+}
 int mhre_version(void) { return 10000; }
 const char* mhre_user_model(void) { return user_model_name(); }
 int mhre_native_eht(void) { return kNativeEht; }

@@ -160,6 +160,10 @@ int mhre_session_destroy(void *session);
 int mhre_run_batch(const MhreFitSpec *fits, int nfits, const MhreOptions *opt, MhreStats *stats);
 
 const char *mhre_last_error(void);
+/* The engine's Philox4x32-10 bijection (host implementation of the device
+ * generator): out = Philox(counter ctr, key key).  No reference counterpart
+ * (the reference draws from cuRAND XORWOW states, src/gpu_mcmc.cu:228-239). */
+void mhre_philox4x32_10(const unsigned int ctr[4], const unsigned int key[2], unsigned int out[4]);
 int mhre_version(void);
 /* Name of the model.cuh compiled into this library build. */
 const char *mhre_user_model(void);
