@@ -132,6 +132,14 @@ def _declare(L):
     L.mhre_session_destroy.argtypes = [vp]
     L.mhre_run_batch.restype = C.c_int
     L.mhre_run_batch.argtypes = [C.POINTER(MhreFitSpec), C.c_int, C.POINTER(MhreOptions), C.POINTER(MhreStats)]
+    L.mhre_eht_data_create.restype = C.c_int
+    L.mhre_eht_data_create.argtypes = [fp, fp, fp, ip, ip] + [C.c_int] * 6 + [C.POINTER(vp)]
+    L.mhre_eht_chi2_grad.restype = C.c_int
+    L.mhre_eht_chi2_grad.argtypes = [vp, vp, C.c_int, C.c_int, vp, vp, vp]
+    L.mhre_eht_gn_system.restype = C.c_int
+    L.mhre_eht_gn_system.argtypes = [vp, vp, C.c_int, C.c_int, vp, vp, vp, vp]
+    L.mhre_eht_data_destroy.restype = C.c_int
+    L.mhre_eht_data_destroy.argtypes = [vp]
 
 
 class EngineError(RuntimeError):

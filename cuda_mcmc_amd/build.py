@@ -68,12 +68,13 @@ def build_library(model_dir, out, native_eht=False, tag=None):
     name = os.path.basename(model_dir.rstrip("/")) + "/model.cuh"
     jobs = [
         ("engine.o", os.path.join(CSRC, "engine.hip"), []),
+        ("grad.o", os.path.join(CSRC, "grad.hip"), []),
         ("generic_%s.o" % tag, os.path.join(CSRC, "generic.hip"),
          ["-I" + model_dir, '-DMHRE_MODEL_NAME="%s"' % name]),
         ("api_%s.o" % ("native" if native_eht else "plain"), os.path.join(CSRC, "api.hip"),
          ["-DMHRE_NATIVE_EHT=1"] if native_eht else []),
     ]
-    with ThreadPoolExecutor(max_workers=3) as ex:
+    with ThreadPoolExecutor(max_workers=4) as ex:
         objs = list(ex.map(lambda j: _obj(*j), jobs))
     os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
     tmp = out + ".tmp"

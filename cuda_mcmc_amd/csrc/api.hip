@@ -25,6 +25,7 @@
 #include "ccalcmodel.h"
 #include "fitdesc.h"
 #include "engine.h"
+#include "grad.h"
 #include "mhre.h"
 #include "mhre_device.h"
 
@@ -96,11 +97,11 @@ struct FitHost {
   std::vector<float> vis;        // 4 per visibility
   std::vector<int> clo;          // 4 per closure
   std::vector<float> clow;       // 4 per closure
-  std::vector<int> clo_range;
+  std::vector<int> tinfo;        // 2 per tile: first closure, max u^2+v^2 (float bits)
   std::vector<float> qcoef;
   std::vector<int> sumh;
   // arena offsets
-  size_t o_vis, o_clo, o_clow, o_crange, o_qcoef;
+  size_t o_vis, o_clo, o_clow, o_tinfo, o_qcoef;
   size_t o_coor, o_dat, o_std2r, o_icoor, o_idat, o_pdescr, o_ivar, o_invar, o_pmint, o_pmaxt, o_beta;
   size_t o_pcur, o_chi2c, o_pstp, o_ncnt, o_nacpt, o_nhist, o_nexch, o_sumh, o_tcur, o_flag, o_ptent, o_ptentn;
   size_t o_pout, o_tout, o_chi2, o_datm, o_chi2m, o_mc;
@@ -219,13 +220,23 @@ int pack_eht(FitHost& h, int idx) {
     h.clow[4 * q + 1] = (float)sin(obs);
     h.clow[4 * q + 2] = s.std2r[nvis + j];
   }
-  h.clo_range.assign((size_t)h.ntile + 1, 0);
+  h.tinfo.assign(2 * ((size_t)h.ntile + 1), 0);
   if (ok) {
     int q = 0;
     for (int t = 0; t <= h.ntile; ++t) {
       while (q < ncph && tile_of[order[q]] < t) ++q;
-      h.clo_range[t] = q;
+      h.tinfo[2 * t] = q;
     }
+  }
+  for (int t = 0; t < h.ntile; ++t) {
+    float mx = 0.0f;
+    for (int i = t * TILE; i < std::min((t + 1) * TILE, h.nvis_k); ++i) {
+      const float u = h.vis[4 * i], v = h.vis[4 * i + 1];
+      mx = std::max(mx, u * u + v * v);
+    }
+    // rounded up so that se * mx bounds every lane's se * (u^2 + v^2)
+    mx = std::nextafter(mx * (1.0f + 1e-6f), INFINITY);
+    memcpy(&h.tinfo[2 * t + 1], &mx, 4);
   }
   return MHRE_OK;
 }
@@ -483,7 +494,7 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
     h.o_vis = A.reserve<float>(h.vis.size());
     h.o_clo = A.reserve<int>(h.clo.size());
     h.o_clow = A.reserve<float>(h.clow.size());
-    h.o_crange = A.reserve<int>(h.clo_range.size());
+    h.o_tinfo = A.reserve<int>(h.tinfo.size());
     h.o_qcoef = A.reserve<float>(h.qcoef.size());
     h.o_coor = A.reserve<float>(s.coor ? s.ncoor : 0);
     h.o_dat = A.reserve<float>(s.ndat);
@@ -554,7 +565,7 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
     d.adapt_target = s.adapt_target > 0.0f ? s.adapt_target : 0.30f;
     d.quad_n = h.quad_n; d.quad_neq = s.ndat;
     d.vis = at<float4>(S, h.o_vis); d.clo = at<int4>(S, h.o_clo); d.clow = at<float4>(S, h.o_clow);
-    d.clo_range = at<int>(S, h.o_crange); d.qcoef = at<float>(S, h.o_qcoef);
+    d.tinfo = at<int2>(S, h.o_tinfo); d.qcoef = at<float>(S, h.o_qcoef);
     d.coor = at<float>(S, h.o_coor); d.dat = at<float>(S, h.o_dat); d.std2r = at<float>(S, h.o_std2r);
     d.icoor = at<int>(S, h.o_icoor); d.idat = at<int>(S, h.o_idat);
     d.pdescr = at<int>(S, h.o_pdescr); d.ivar = at<int>(S, h.o_ivar); d.invar = at<int>(S, h.o_invar);
@@ -575,7 +586,7 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
     CHK(h2d(S, h.o_vis, h.vis.data(), h.vis.size()));
     CHK(h2d(S, h.o_clo, h.clo.data(), h.clo.size()));
     CHK(h2d(S, h.o_clow, h.clow.data(), h.clow.size()));
-    CHK(h2d(S, h.o_crange, h.clo_range.data(), h.clo_range.size()));
+    CHK(h2d(S, h.o_tinfo, h.tinfo.data(), h.tinfo.size()));
     CHK(h2d(S, h.o_qcoef, h.qcoef.data(), h.qcoef.size()));
     CHK(h2d(S, h.o_coor, s.coor, s.coor ? s.ncoor : 0));
     CHK(h2d(S, h.o_dat, s.dat, s.ndat));
@@ -861,6 +872,81 @@ MhreFitSpec legacy_spec(float* coor, float* dat, float* std2r, int* icoor, int* 
   return s;
 }
 
+// ----------------------------------------------- packed data for gradients
+
+struct EhtData {
+  FitHost h;
+  char* dev = nullptr;
+  FitDesc* d_fd = nullptr;
+};
+
+void destroy_eht_data(EhtData* E) {
+  if (!E) return;
+  if (E->dev) hipFree(E->dev);
+  delete E;
+}
+
+int eht_data_create(const float* coor, const float* dat, const float* std2r, const int* icoor, const int* idat,
+                    int imodel, int ncoor, int ndat, int nicoor, int nidat, int nptot, EhtData** out) {
+  *out = nullptr;
+  (void)hipGetLastError();
+  if (imodel != 1) FAIL(MHRE_EINVAL, "gradient kernels implement the 9-parameter model (imodel 1) only");
+  if (!coor || !dat || !std2r || !idat) FAIL(MHRE_EINVAL, "coor, dat, std2r and idat are required");
+  if (ndat <= 0) FAIL(MHRE_EINVAL, "ndat must be positive");
+  if (nptot < 9 || nptot > MAXP) FAIL(MHRE_EINVAL, "nptot = %d not in [9, %d]", nptot, MAXP);
+  EhtData* E = new EhtData();
+  struct Guard {
+    EhtData*& e;
+    ~Guard() { if (e) destroy_eht_data(e); }
+  } guard{E};
+  FitHost& h = E->h;
+  MhreFitSpec& s = h.s;
+  s.model = MHRE_MODEL_SGRA9;
+  s.imodel = imodel;
+  s.coor = const_cast<float*>(coor); s.dat = const_cast<float*>(dat); s.std2r = const_cast<float*>(std2r);
+  s.icoor = const_cast<int*>(icoor); s.idat = const_cast<int*>(idat);
+  s.ncoor = ncoor; s.ndat = ndat; s.nicoor = nicoor; s.nidat = nidat; s.nptot = nptot;
+  h.kind = K_SGRA9;
+  CHK(pack_eht(h, 0));
+  Arena A;
+  const size_t o_vis = A.reserve<float>(h.vis.size());
+  const size_t o_clo = A.reserve<int>(h.clo.size());
+  const size_t o_clow = A.reserve<float>(h.clow.size());
+  const size_t o_tinfo = A.reserve<int>(h.tinfo.size());
+  const size_t o_fd = A.reserve<FitDesc>(1);
+  HIPCHK(hipMalloc((void**)&E->dev, A.bytes));
+  FitDesc d{};
+  d.model = MODEL_SGRA9;
+  d.imodel = imodel;
+  d.nptot = nptot;
+  d.ndat = ndat;
+  d.nvis = h.nvis_k; d.ncph = h.ncph; d.ntile = h.ntile; d.ring_ok = h.ring_ok;
+  d.vis = reinterpret_cast<const float4*>(E->dev + o_vis);
+  d.clo = reinterpret_cast<const int4*>(E->dev + o_clo);
+  d.clow = reinterpret_cast<const float4*>(E->dev + o_clow);
+  d.tinfo = reinterpret_cast<const int2*>(E->dev + o_tinfo);
+  E->d_fd = reinterpret_cast<FitDesc*>(E->dev + o_fd);
+  HIPCHK(hipMemcpy(E->dev + o_vis, h.vis.data(), h.vis.size() * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(E->dev + o_clo, h.clo.data(), h.clo.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(E->dev + o_clow, h.clow.data(), h.clow.size() * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(E->dev + o_tinfo, h.tinfo.data(), h.tinfo.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(E->d_fd, &d, sizeof(d), hipMemcpyHostToDevice));
+  h.d = d;
+  // the caller's arrays are not referenced after this point
+  s.coor = nullptr; s.dat = nullptr; s.std2r = nullptr; s.icoor = nullptr; s.idat = nullptr;
+  *out = E;
+  E = nullptr;
+  return MHRE_OK;
+}
+
+int eht_check_batch(const EhtData* E, const float* P, int n, int ldp, const double* chi2) {
+  if (!E) FAIL(MHRE_EINVAL, "null data handle");
+  if (n < 0) FAIL(MHRE_EINVAL, "n = %d is negative", n);
+  if (n > 0 && (!P || !chi2)) FAIL(MHRE_EINVAL, "params and chi2 are required");
+  if (ldp < 9 || ldp > MAXP) FAIL(MHRE_EINVAL, "ldp = %d not in [9, %d]", ldp, MAXP);
+  return MHRE_OK;
+}
+
 }  // namespace
 
 // ===================================================================== C ABI
@@ -1048,6 +1134,40 @@ int list_gpus(char* buf, int buflen) {
   }
   snprintf(buf, (size_t)buflen, "%s", out.c_str());
   return n;
+}
+
+int mhre_eht_data_create(const float* coor, const float* dat, const float* std2r, const int* icoor,
+                         const int* idat, int imodel, int ncoor, int ndat, int nicoor, int nidat, int nptot,
+                         void** handle) {
+  if (!handle) FAIL(MHRE_EINVAL, "null handle pointer");
+  EhtData* E = nullptr;
+  const int rc = eht_data_create(coor, dat, std2r, icoor, idat, imodel, ncoor, ndat, nicoor, nidat, nptot, &E);
+  *handle = E;
+  return rc;
+}
+
+int mhre_eht_chi2_grad(void* handle, const float* params, int n, int ldp, double* chi2, float* grad,
+                       void* stream) {
+  (void)hipGetLastError();
+  const EhtData* E = static_cast<const EhtData*>(handle);
+  CHK(eht_check_batch(E, params, n, ldp, chi2));
+  HIPCHK(launch_eht_grad9(E->d_fd, params, n, ldp, chi2, grad, static_cast<hipStream_t>(stream)));
+  return MHRE_OK;
+}
+
+int mhre_eht_gn_system(void* handle, const float* params, int n, int ldp, double* chi2, double* jtr,
+                       double* jtj, void* stream) {
+  (void)hipGetLastError();
+  const EhtData* E = static_cast<const EhtData*>(handle);
+  CHK(eht_check_batch(E, params, n, ldp, chi2));
+  if (n > 0 && (!jtr || !jtj)) FAIL(MHRE_EINVAL, "jtr and jtj are required");
+  HIPCHK(launch_eht_gn9(E->d_fd, params, n, ldp, chi2, jtr, jtj, static_cast<hipStream_t>(stream)));
+  return MHRE_OK;
+}
+
+int mhre_eht_data_destroy(void* handle) {
+  destroy_eht_data(static_cast<EhtData*>(handle));
+  return MHRE_OK;
 }
 
 }  // extern "C"

@@ -45,7 +45,8 @@ __device__ inline void ring_coefs(double Rex, double rq, double fade, double d, 
 }
 
 // Ring visibility Fr = Fex - Fin at rotated coordinates (U, V).
-// uniform_small: caller guarantees ye <= 64 on every active lane.
+// any_large == false: the caller guarantees ye <= 64 on every lane, so only
+// the polynomial branch of B, C is evaluated.
 __device__ inline float2 ring_vis(const RingInv& q, float U, float V, bool any_large) {
   const float rho2 = fmaf(U, U, V * V);
   const float ye = q.se * rho2;
@@ -104,9 +105,6 @@ struct Sgra9 {
     return q;
   }
 
-  __device__ static float max_y(const Inv& q, float u, float v) {
-    return q.r.se * fmaf(u, u, v * v);   // rotation preserves rho
-  }
 
   __device__ static float2 vis(const Inv& q, float u, float v, bool any_large) {
     const float U = fmaf(u, q.cth, v * q.sth);
@@ -114,7 +112,7 @@ struct Sgra9 {
     const float2 F = ring_vis(q.r, U, V, any_large);
     float2 out;
     if (q.r.gauss) {
-      const float ga = exp2f(-fmaf(q.gaU, U * U, q.gaV * (V * V)));
+      const float ga = __builtin_amdgcn_exp2f(-fmaf(q.gaU, U * U, q.gaV * (V * V)));
       float sg, cgs;
       sincos_fast(q.kg * U, &sg, &cgs);
       const float gw = q.r.cg * ga;
@@ -168,9 +166,6 @@ struct Sgra13 {
     return q;
   }
 
-  __device__ static float max_y(const Inv& q, float u, float v) {
-    return q.r.se * fmaf(u, u, v * v);
-  }
 
   __device__ static float2 vis(const Inv& q, float u, float v, bool any_large) {
     float2 out = make_float2(0.0f, 0.0f);
@@ -184,7 +179,7 @@ struct Sgra13 {
     if (q.r.gauss) {
       const float Ub = fmaf(u, q.cb, v * q.sb);
       const float Vb = fmaf(v, q.cb, -u * q.sb);
-      const float ga = exp2f(-fmaf(q.gaU, Ub * Ub, q.gaV * (Vb * Vb)));
+      const float ga = __builtin_amdgcn_exp2f(-fmaf(q.gaU, Ub * Ub, q.gaV * (Vb * Vb)));
       float sg, cgs;
       sincos_fast(fmaf(q.kgu, u, q.kgv * v), &sg, &cgs);
       const float gw = q.r.cg * ga;

@@ -87,7 +87,7 @@ struct EhtView {
   const GLOBAL float4* vis;
   const GLOBAL int4* clo;
   const GLOBAL float4* clow;
-  const GLOBAL int* clo_range;
+  const GLOBAL int2* tinfo;
   GLOBAL float* datm;
   GLOBAL float* chi2m;
   long long ndatm;
@@ -96,7 +96,7 @@ struct EhtView {
 
 __device__ inline EhtView eht_view(const FitDesc& f) {
   EhtView v;
-  v.vis = gptr(f.vis); v.clo = gptr(f.clo); v.clow = gptr(f.clow); v.clo_range = gptr(f.clo_range);
+  v.vis = gptr(f.vis); v.clo = gptr(f.clo); v.clow = gptr(f.clow); v.tinfo = gptr(f.tinfo);
   v.datm = gptr(f.datm); v.chi2m = gptr(f.chi2m); v.ndatm = f.ndatm;
   v.nvis = f.nvis; v.ncph = f.ncph; v.ntile = f.ntile; v.ring_ok = f.ring_ok;
   v.ndat = f.ndat; v.nchain = f.nchain;
@@ -142,22 +142,23 @@ __device__ double eht_chi2_wave(const EhtView& v, const typename M::Inv& q, floa
   // All loads below are unconditional with clamped (in-bounds) indices so the
   // compiler can keep them in flight across the model evaluation.
   float4 dn = v.vis[min(lane, lastv)];
+  // per tile: (first closure after it, max rho^2); carried one tile ahead
   int c0 = 0;
-  int c1 = clos ? v.clo_range[1] : 0;
+  int2 ti = v.tinfo[min(1, v.ntile)];
+  float r2cur = __int_as_float(v.tinfo[0].y);
   for (int t = 0; t < v.ntile; ++t) {
     const int i = t * TILE + lane;
     const float4 d = dn;
     dn = v.vis[min(i + TILE, lastv)];
-    const int c1n = clos ? v.clo_range[min(t + 2, v.ntile)] : 0;
+    const int2 tin = v.tinfo[min(t + 2, v.ntile)];
+    const int c1 = clos ? ti.x : 0;
     const int cf = c0 + lane;
-    int4 ix = make_int4(0, 0, 0, 0);
-    float4 ow = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    if (clos) {
-      ix = v.clo[min(cf, lastc)];
-      ow = v.clow[min(cf, lastc)];
-    }
+    // (clo, clow and tinfo are allocated, at least one element, for every
+    // native fit, so these loads are always in bounds)
+    int4 ix = v.clo[min(cf, lastc)];
+    float4 ow = v.clow[min(cf, lastc)];
     const bool live = i < v.nvis;
-    const bool any_large = __any(live && M::max_y(q, d.x, d.y) > 64.0f);
+    const bool any_large = q.r.se * r2cur > 64.0f;
     const float2 V = M::vis(q, d.x, d.y, any_large);
     const float amp = __builtin_amdgcn_sqrtf(fmaf(V.x, V.x, V.y * V.y));
     const float r = d.z - amp;
@@ -193,7 +194,8 @@ __device__ double eht_chi2_wave(const EhtView& v, const typename M::Inv& q, floa
       ring_fence();
     }
     c0 = c1;
-    c1 = c1n;
+    r2cur = __int_as_float(ti.y);
+    ti = tin;
   }
   if (v.ncph && !v.ring_ok) {
     // General closure layout: recompute the three model visibilities.

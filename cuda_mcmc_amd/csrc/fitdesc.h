@@ -47,7 +47,8 @@ struct FitDesc {
   const float4* vis;       // [nvis] (u, v, amp, 1/sigma^2) for the native kernel
   const int4* clo;         // [ncph] vis indices (i, j, k, datum index) sorted by ring tile
   const float4* clow;      // [ncph] (cos obs, sin obs, 1/sigma^2, 0)
-  const int* clo_range;    // [ntile + 1] closures handled after each tile
+  const int2* tinfo;       // [ntile + 1] per tile: first closure handled after it,
+                           // and max u^2+v^2 of its visibilities (float bits)
   const float* qcoef;      // MODEL_QUADSYS packed coefficients
   const float* coor; const float* dat; const float* std2r;
   const int* icoor; const int* idat;

@@ -1,0 +1,395 @@
+// chi^2 gradient and Gauss-Newton normal equations of the native 9-parameter
+// ring+Gaussian model (examples/sgra_model_fit/model.cuh:322-462), for batches
+// of parameter vectors.  No reference counterpart: the reference samples
+// without derivatives.
+//
+// One wave64 per parameter vector; lanes stride over the data.
+//   * The model of one visibility is a function of the fourteen fp32
+//     invariants that Sgra9::prepare builds from the 9 parameters.  Each datum
+//     back-propagates its residual's adjoint into those invariants (reverse
+//     mode, fp32 per datum, fp64 accumulation per lane).
+//   * The invariants' 14 x 9 Jacobian comes from a float64 forward-mode pass
+//     over the same formulas: lane j carries the tangent along parameter j.
+//   * gradient = J_inv^T * (sum of the invariant adjoints).
+//   * k_eht_gn9 forms per datum the residual row dr/dp = J_inv^T dr/dinv and
+//     accumulates J^T J (45 packed entries) and J^T r in float64, with
+//     r_i = (dat_i - model_i) sqrt(w_i) (closures: the wrapped phase residual),
+//     so chi^2 = |r|^2 and grad chi^2 = 2 J^T r.
+// Bessel derivatives: with B(y) = J1(x)/x, C(y) = J2(x)/x^2, D(y) = J3(x)/x^3
+// and y = x^2, dB/dy = -C/2 and dC/dy = -D/2 (from d/dx[x^-n J_n] = -x^-n J_n+1).
+#include <hip/hip_runtime.h>
+#include "mhre_device.h"
+#include "eht_model.h"
+#include "fitdesc.h"
+#include "grad.h"
+
+namespace mhre {
+
+constexpr int NINV = 14;   // cth sth se rq2 kBe kCe kBi kCi kd cr cg gaU gaV kg
+constexpr int NP9 = 9;
+constexpr int NJTJ = NP9 * (NP9 + 1) / 2;
+
+// ------------------------------------------------- float64 forward mode
+
+struct Dual {
+  double v, d;
+};
+__device__ inline Dual dc(double c) { return Dual{c, 0.0}; }
+__device__ inline Dual operator+(Dual a, Dual b) { return Dual{a.v + b.v, a.d + b.d}; }
+__device__ inline Dual operator-(Dual a, Dual b) { return Dual{a.v - b.v, a.d - b.d}; }
+__device__ inline Dual operator*(Dual a, Dual b) { return Dual{a.v * b.v, a.d * b.v + a.v * b.d}; }
+__device__ inline Dual operator*(double c, Dual a) { return Dual{c * a.v, c * a.d}; }
+__device__ inline Dual operator/(Dual a, Dual b) {
+  const double q = a.v / b.v;
+  return Dual{q, (a.d - q * b.d) / b.v};
+}
+__device__ inline Dual dsin(Dual a) { return Dual{sin(a.v), cos(a.v) * a.d}; }
+__device__ inline Dual dcos(Dual a) { return Dual{cos(a.v), -sin(a.v) * a.d}; }
+
+// The invariants of Sgra9::prepare (same formulas, float64 duals).
+// out[]: cth sth se rq2 kBe kCe kBi kCi kd cr cg gaU gaV kg
+__device__ inline void sgra9_inv_dual(const Dual* p, Dual* out) {
+  const Dual Zsp = p[0], Re = p[1], rq = p[2], ecc = p[3], fade = p[4];
+  const Dual gax = p[5], aq = p[6], gq = p[7], th = p[8];
+  const Dual Rex = UAS2NRAD * Re;
+  const Dual Rin = rq * Rex;
+  const Dual d = ecc * (Rex - Rin);
+  const Dual one = dc(1.0);
+  const Dual den = (Rex * Rex - Rin * Rin * (one + d / Rex)) * fade + (Rex * Rex - Rin * Rin * (one - d / Rex));
+  const Dual h = dc(2.0 / D_PI) / den;
+  const Dual h0 = fade * h;
+  out[0] = dcos(th);
+  out[1] = dsin(th);
+  out[2] = (2.0 * D_PI * Rex) * (2.0 * D_PI * Rex);
+  out[3] = rq * rq;
+  out[4] = D_PI * ((h + h0) * Rex * Rex);
+  out[5] = (-2.0 * D_PI * D_PI) * ((h - h0) * Rex * Rex * Rex);
+  out[6] = D_PI * ((h + h0 - d * (h - h0) / Rex) * Rin * Rin);
+  out[7] = (-2.0 * D_PI * D_PI) * ((h - h0) * rq * Rin * Rin * Rin);
+  out[8] = (2.0 * D_PI) * d;
+  const bool nog = (p[5].v == 0.0) || (p[6].v == 0.0) || (p[7].v == 0.0);
+  const Dual gsy = gax * Rex, gsx = aq * gsy;
+  out[9] = nog ? Zsp : Zsp * (one - gq);
+  out[10] = nog ? dc(0.0) : Zsp * gq;
+  out[11] = (GCOEF * LOG2E) * (gsx * gsx);
+  out[12] = (GCOEF * LOG2E) * (gsy * gsy);
+  out[13] = (-2.0 * D_PI) * (Rin - d);
+}
+
+// ------------------------------------------------------ per-datum model
+
+// B, C and D at y = x^2 (polynomials for y <= 64; above, Hankel asymptotics
+// for B and C and D = (4C - B)/y from J3 = 4 J2/x - J1).
+__device__ inline void bessel_BCD(float y, float* B, float* C, float* D) {
+  if (y <= 64.0f) {
+    const float t = fmaf(y, 0.03125f, -1.0f);
+    *B = horner(BESSEL_B, t);
+    *C = horner(BESSEL_C, t);
+    *D = horner(BESSEL_D, t);
+  } else {
+    bessel_BC_asym(y, B, C);
+    *D = (4.0f * *C - *B) * __builtin_amdgcn_rcpf(y);
+  }
+}
+
+struct Fwd9 {
+  float u, v, U, W, rho2, ye, Be, Ce, De, Bi, Ci, Di, sd, cd, Fx, Fy, ga, sg, cgs, Vx, Vy;
+};
+
+// Forward model of one visibility, keeping what the backward pass needs.
+// Same arithmetic as Sgra9::vis with the full-range Bessel branch.
+__device__ inline Fwd9 sgra9_fwd(const Sgra9::Inv& q, float u, float v) {
+  Fwd9 f;
+  f.u = u;
+  f.v = v;
+  f.U = fmaf(u, q.cth, v * q.sth);
+  f.W = fmaf(v, q.cth, -u * q.sth);
+  f.rho2 = fmaf(f.U, f.U, f.W * f.W);
+  f.ye = q.r.se * f.rho2;
+  const float yi = q.r.rq2 * f.ye;
+  bessel_BCD(f.ye, &f.Be, &f.Ce, &f.De);
+  bessel_BCD(yi, &f.Bi, &f.Ci, &f.Di);
+  const float ree = q.r.kBe * f.Be;
+  const float ime = q.r.kCe * f.U * f.Ce;
+  const float rei = q.r.kBi * f.Bi;
+  const float imi = q.r.kCi * f.U * f.Ci;
+  sincos_fast(q.r.kd * f.U, &f.sd, &f.cd);
+  f.Fx = ree - (rei * f.cd - imi * f.sd);
+  f.Fy = ime - (rei * f.sd + imi * f.cd);
+  if (q.r.gauss) {
+    f.ga = __builtin_amdgcn_exp2f(-fmaf(q.gaU, f.U * f.U, q.gaV * (f.W * f.W)));
+    sincos_fast(q.kg * f.U, &f.sg, &f.cgs);
+    const float gw = q.r.cg * f.ga;
+    f.Vx = fmaf(q.r.cr, f.Fx, gw * f.cgs);
+    f.Vy = fmaf(q.r.cr, f.Fy, gw * f.sg);
+  } else {
+    f.ga = f.sg = f.cgs = 0.0f;
+    f.Vx = q.r.cr * f.Fx;
+    f.Vy = q.r.cr * f.Fy;
+  }
+  return f;
+}
+
+// Reverse pass of sgra9_fwd: adds (gx, gy) . d(Vx, Vy)/d(invariant m) to a[m].
+template <class A>
+__device__ inline void sgra9_bwd(const Sgra9::Inv& q, const Fwd9& f, float gx, float gy, A* a) {
+  constexpr float LN2 = 0.693147180559945309f;
+  a[9] += (A)(gx * f.Fx + gy * f.Fy);
+  const float aFx = q.r.cr * gx, aFy = q.r.cr * gy;
+  float aU = 0.0f, aW = 0.0f;
+  if (q.r.gauss) {
+    const float gw = q.r.cg * f.ga;
+    const float agw = gx * f.cgs + gy * f.sg;
+    const float aphg = gw * (gy * f.cgs - gx * f.sg);
+    a[10] += (A)(agw * f.ga);
+    const float aE = -LN2 * agw * gw;
+    a[11] += (A)(aE * f.U * f.U);
+    a[12] += (A)(aE * f.W * f.W);
+    aU += 2.0f * aE * q.gaU * f.U + aphg * q.kg;
+    aW += 2.0f * aE * q.gaV * f.W;
+    a[13] += (A)(aphg * f.U);
+  }
+  const float rei = q.r.kBi * f.Bi, imi = q.r.kCi * f.U * f.Ci;
+  const float arei = -aFx * f.cd - aFy * f.sd;
+  const float aimi = aFx * f.sd - aFy * f.cd;
+  const float asd = aFx * imi - aFy * rei;
+  const float acd = -aFx * rei - aFy * imi;
+  const float aphd = asd * f.cd - acd * f.sd;
+  a[8] += (A)(aphd * f.U);
+  aU += aphd * q.r.kd;
+  a[4] += (A)(aFx * f.Be);
+  const float aBe = aFx * q.r.kBe;
+  a[5] += (A)(aFy * f.U * f.Ce);
+  aU += aFy * q.r.kCe * f.Ce;
+  const float aCe = aFy * q.r.kCe * f.U;
+  a[6] += (A)(arei * f.Bi);
+  const float aBi = arei * q.r.kBi;
+  a[7] += (A)(aimi * f.U * f.Ci);
+  aU += aimi * q.r.kCi * f.Ci;
+  const float aCi = aimi * q.r.kCi * f.U;
+  float aye = -0.5f * (aBe * f.Ce + aCe * f.De);
+  const float ayi = -0.5f * (aBi * f.Ci + aCi * f.Di);
+  a[3] += (A)(ayi * f.ye);
+  aye += ayi * q.r.rq2;
+  a[2] += (A)(aye * f.rho2);
+  const float arho2 = aye * q.r.se;
+  aU += 2.0f * f.U * arho2;
+  aW += 2.0f * f.W * arho2;
+  a[0] += (A)(aU * f.u + aW * f.v);
+  a[1] += (A)(aU * f.v - aW * f.u);
+}
+
+// d arg(V) / d(Vx, Vy) scaled by s.
+__device__ inline float2 darg(const Fwd9& f, float s) {
+  const float m2 = fmaxf(fmaf(f.Vx, f.Vx, f.Vy * f.Vy), 1e-30f);
+  const float k = s * __builtin_amdgcn_rcpf(m2);
+  return make_float2(-f.Vy * k, f.Vx * k);
+}
+
+// Closure residual dd = wrap(obs - (pa + pb - pc)) as in closure_term.
+__device__ inline float closure_resid(const Fwd9& a, const Fwd9& b, const Fwd9& e, float4 ow) {
+  const float abr = a.Vx * b.Vx - a.Vy * b.Vy, abi = a.Vx * b.Vy + a.Vy * b.Vx;
+  const float br = abr * e.Vx + abi * e.Vy, bi = abi * e.Vx - abr * e.Vy;
+  const float xr = ow.x * br + ow.y * bi;
+  const float xi = ow.y * br - ow.x * bi;
+  return atan2_fast(xi, xr);
+}
+
+// ----------------------------------------------------------------- kernels
+
+struct GradArgs {
+  const FitDesc* fd;   // one fit's packed data (device)
+  const float* P;      // [n][ldp] parameter vectors (device)
+  int n, ldp;
+  double* chi2;        // [n]
+  float* grad;         // [n][ldp] or null
+  double* jtr;         // [n][9]   (k_eht_gn9)
+  double* jtj;         // [n][45]  (k_eht_gn9), row-major upper triangle
+};
+
+// Lane j < 9 computes column j of d(inv)/dp into jac[m * 9 + j] (LDS).
+__device__ inline void inv_jacobian(float pl, double* jac) {
+  const int lane = lane_id();
+  Dual pd[NP9];
+#pragma unroll
+  for (int i = 0; i < NP9; ++i) pd[i] = Dual{(double)bcast(pl, i), i == lane ? 1.0 : 0.0};
+  Dual inv[NINV];
+  sgra9_inv_dual(pd, inv);
+  if (lane < NP9) {
+#pragma unroll
+    for (int m = 0; m < NINV; ++m) jac[m * NP9 + lane] = inv[m].d;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront", "local");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront", "local");
+}
+
+__global__ void __launch_bounds__(256) k_eht_grad9(GradArgs g) {
+  __shared__ double jac_all[4][NINV * NP9];
+  const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int w = blockIdx.x * 4 + wid;
+  if (w >= g.n) return;
+  const int lane = lane_id();
+  const FitDesc& f = *g.fd;
+  const float pl = lane < NP9 ? g.P[(size_t)w * g.ldp + lane] : 0.0f;
+  float p[NP9];
+#pragma unroll
+  for (int j = 0; j < NP9; ++j) p[j] = bcast(pl, j);
+  const Sgra9::Inv q = Sgra9::prepare(p);
+  double* jac = jac_all[wid];
+  if (g.grad) inv_jacobian(pl, jac);
+
+  const int nvr = f.ndat - f.ncph;
+  double chi = 0.0;
+  double a[NINV];
+#pragma unroll
+  for (int m = 0; m < NINV; ++m) a[m] = 0.0;
+  for (int i = lane; i < nvr; i += 64) {
+    const float4 d = f.vis[i];
+    const Fwd9 fw = sgra9_fwd(q, d.x, d.y);
+    const float amp = __builtin_amdgcn_sqrtf(fmaf(fw.Vx, fw.Vx, fw.Vy * fw.Vy));
+    const float r = d.z - amp;
+    chi += (double)(r * r * d.w);
+    if (g.grad) {
+      const float s = -2.0f * r * d.w * __builtin_amdgcn_rcpf(fmaxf(amp, 1e-30f));
+      sgra9_bwd(q, fw, s * fw.Vx, s * fw.Vy, a);
+    }
+  }
+  for (int c = lane; c < f.ncph; c += 64) {
+    const int4 ix = f.clo[c];
+    const float4 ow = f.clow[c];
+    const float4 da = f.vis[ix.x], db = f.vis[ix.y], de = f.vis[ix.z];
+    const Fwd9 fa = sgra9_fwd(q, da.x, da.y);
+    const Fwd9 fb = sgra9_fwd(q, db.x, db.y);
+    const Fwd9 fe = sgra9_fwd(q, de.x, de.y);
+    const float dd = closure_resid(fa, fb, fe, ow);
+    chi += (double)(dd * dd * ow.z);
+    if (g.grad) {
+      // dd = obs - pa - pb + pc: d(dd^2 w)/dpa = -2 dd w
+      const float s = -2.0f * dd * ow.z;
+      const float2 ga = darg(fa, s), gb = darg(fb, s), ge = darg(fe, -s);
+      sgra9_bwd(q, fa, ga.x, ga.y, a);
+      sgra9_bwd(q, fb, gb.x, gb.y, a);
+      sgra9_bwd(q, fe, ge.x, ge.y, a);
+    }
+  }
+  chi = wave_sum(chi);
+  if (lane == 0) g.chi2[w] = chi;
+  if (!g.grad) return;
+#pragma unroll
+  for (int m = 0; m < NINV; ++m) a[m] = wave_sum(a[m]);
+  if (lane < g.ldp) {
+    double gj = 0.0;
+    if (lane < NP9) {
+#pragma unroll
+      for (int m = 0; m < NINV; ++m) gj += a[m] * jac[m * NP9 + lane];
+    }
+    g.grad[(size_t)w * g.ldp + lane] = (float)gj;
+  }
+}
+
+// Adds the residual row of one datum: dr/dp = J_inv^T dr/dinv.
+__device__ inline void gn_accumulate(const float* rinv, float r, const double* jac, double* jtr, double* jtj) {
+  float row[NP9];
+#pragma unroll
+  for (int j = 0; j < NP9; ++j) {
+    double s = 0.0;
+#pragma unroll
+    for (int m = 0; m < NINV; ++m) s += (double)rinv[m] * jac[m * NP9 + j];
+    row[j] = (float)s;
+  }
+  int k = 0;
+#pragma unroll
+  for (int i = 0; i < NP9; ++i) {
+    jtr[i] += (double)row[i] * (double)r;
+#pragma unroll
+    for (int j = i; j < NP9; ++j) jtj[k++] += (double)row[i] * (double)row[j];
+  }
+}
+
+__global__ void __launch_bounds__(64) k_eht_gn9(GradArgs g) {
+  __shared__ double jac[NINV * NP9];
+  const int w = blockIdx.x;
+  if (w >= g.n) return;
+  const int lane = lane_id();
+  const FitDesc& f = *g.fd;
+  const float pl = lane < NP9 ? g.P[(size_t)w * g.ldp + lane] : 0.0f;
+  float p[NP9];
+#pragma unroll
+  for (int j = 0; j < NP9; ++j) p[j] = bcast(pl, j);
+  const Sgra9::Inv q = Sgra9::prepare(p);
+  inv_jacobian(pl, jac);
+
+  const int nvr = f.ndat - f.ncph;
+  double chi = 0.0;
+  double jtr[NP9], jtj[NJTJ];
+#pragma unroll
+  for (int i = 0; i < NP9; ++i) jtr[i] = 0.0;
+#pragma unroll
+  for (int i = 0; i < NJTJ; ++i) jtj[i] = 0.0;
+  for (int i = lane; i < nvr; i += 64) {
+    const float4 d = f.vis[i];
+    const Fwd9 fw = sgra9_fwd(q, d.x, d.y);
+    const float amp = __builtin_amdgcn_sqrtf(fmaf(fw.Vx, fw.Vx, fw.Vy * fw.Vy));
+    const float sw = __builtin_amdgcn_sqrtf(d.w);
+    const float r = (d.z - amp) * sw;
+    chi += (double)r * (double)r;
+    // dr/dV = -sqrt(w) V / amp
+    const float s = -sw * __builtin_amdgcn_rcpf(fmaxf(amp, 1e-30f));
+    float rinv[NINV];
+#pragma unroll
+    for (int m = 0; m < NINV; ++m) rinv[m] = 0.0f;
+    sgra9_bwd(q, fw, s * fw.Vx, s * fw.Vy, rinv);
+    gn_accumulate(rinv, r, jac, jtr, jtj);
+  }
+  for (int c = lane; c < f.ncph; c += 64) {
+    const int4 ix = f.clo[c];
+    const float4 ow = f.clow[c];
+    const float4 da = f.vis[ix.x], db = f.vis[ix.y], de = f.vis[ix.z];
+    const Fwd9 fa = sgra9_fwd(q, da.x, da.y);
+    const Fwd9 fb = sgra9_fwd(q, db.x, db.y);
+    const Fwd9 fe = sgra9_fwd(q, de.x, de.y);
+    const float sw = __builtin_amdgcn_sqrtf(ow.z);
+    const float r = closure_resid(fa, fb, fe, ow) * sw;
+    chi += (double)r * (double)r;
+    // r = sqrt(w) (obs - pa - pb + pc)
+    const float2 ga = darg(fa, -sw), gb = darg(fb, -sw), ge = darg(fe, sw);
+    float rinv[NINV];
+#pragma unroll
+    for (int m = 0; m < NINV; ++m) rinv[m] = 0.0f;
+    sgra9_bwd(q, fa, ga.x, ga.y, rinv);
+    sgra9_bwd(q, fb, gb.x, gb.y, rinv);
+    sgra9_bwd(q, fe, ge.x, ge.y, rinv);
+    gn_accumulate(rinv, r, jac, jtr, jtj);
+  }
+  chi = wave_sum(chi);
+#pragma unroll
+  for (int i = 0; i < NP9; ++i) jtr[i] = wave_sum(jtr[i]);
+#pragma unroll
+  for (int i = 0; i < NJTJ; ++i) jtj[i] = wave_sum(jtj[i]);
+  if (lane == 0) {
+    g.chi2[w] = chi;
+#pragma unroll
+    for (int i = 0; i < NP9; ++i) g.jtr[(size_t)w * NP9 + i] = jtr[i];
+#pragma unroll
+    for (int i = 0; i < NJTJ; ++i) g.jtj[(size_t)w * NJTJ + i] = jtj[i];
+  }
+}
+
+hipError_t launch_eht_grad9(const FitDesc* fd, const float* P, int n, int ldp, double* chi2, float* grad,
+                            hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  GradArgs g{fd, P, n, ldp, chi2, grad, nullptr, nullptr};
+  k_eht_grad9<<<(n + 3) / 4, 256, 0, s>>>(g);
+  return hipGetLastError();
+}
+
+hipError_t launch_eht_gn9(const FitDesc* fd, const float* P, int n, int ldp, double* chi2, double* jtr,
+                          double* jtj, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  GradArgs g{fd, P, n, ldp, chi2, nullptr, jtr, jtj};
+  k_eht_gn9<<<n, 64, 0, s>>>(g);
+  return hipGetLastError();
+}
+
+}  // namespace mhre

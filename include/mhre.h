@@ -170,6 +170,32 @@ const char *mhre_user_model(void);
 /* 1 if imodel 1/2 use the native fused kernels in this build. */
 int mhre_native_eht(void);
 
+/* ------------------------------------------------------------ Part 3 */
+
+/* chi^2 gradients of the 9-parameter ring+Gaussian model (imodel 1) for
+ * batches of parameter vectors.  No reference counterpart (the reference
+ * samples without derivatives); the chi^2 is the one of calc_modchi2
+ * (src/mcmc.h:23-27) for the same data arrays.  params, chi2, grad, jtr and
+ * jtj are DEVICE pointers; stream is a hipStream_t (NULL: the null stream);
+ * the calls are asynchronous to the host. */
+
+/* Pack one data set (the calc_modchi2 data arrays) on the current device. */
+int mhre_eht_data_create(const float *coor, const float *dat, const float *std2r,
+                         const int *icoor, const int *idat, int imodel,
+                         int ncoor, int ndat, int nicoor, int nidat, int nptot,
+                         void **handle);
+/* chi2[n] and, if grad != NULL, grad[n][ldp] = d chi^2 / d params for the
+ * parameter vectors params[n][ldp] (9 <= ldp <= 32; entries past 9 get 0). */
+int mhre_eht_chi2_grad(void *handle, const float *params, int n, int ldp,
+                       double *chi2, float *grad, void *stream);
+/* Gauss-Newton normal equations of the weighted residuals r (amplitudes:
+ * (dat - |V|) sqrt(std2r); closures: the wrapped phase residual times
+ * sqrt(std2r)), so chi^2 = |r|^2: chi2[n], jtr[n][9] = J^T r and the upper
+ * triangle of J^T J row by row, jtj[n][45]. */
+int mhre_eht_gn_system(void *handle, const float *params, int n, int ldp,
+                       double *chi2, double *jtr, double *jtj, void *stream);
+int mhre_eht_data_destroy(void *handle);
+
 #ifdef __cplusplus
 }
 #endif
