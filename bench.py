@@ -16,8 +16,15 @@ Also reported:
                 visibility-eval, 15 per closure term, 40 per chain setup)
                 over its HIP-event duration, against the 157.3 TFLOP/s peak;
   cpu_baseline  the float64 C/OpenMP oracle port on the host cores;
-  batched       64 uneven fits in one session vs the same fits one session
-                at a time.
+  batched       64 uneven fits (config 3 shapes) in one persistent session,
+                launch form picked by the autotuner, vs the same fits one
+                session at a time;
+  adjoint       chi^2 + gradient of 4096 parameter points against 20000
+                visibilities + 6000 closure phases (fp32, HIP events), and a
+                Gauss-Newton polish + Laplace covariance of short fits;
+  quadsys       1024 systems of 8 quadratic equations, 16 x 32 chains each
+                (524288 chains), 3000 + 10000 iterations in one session, then
+                Gauss-Newton polish; success = residual norm < 1e-5.
 
 Prints exactly one JSON line on stdout; progress goes to stderr.
 """
@@ -35,6 +42,9 @@ sys.path.insert(0, ROOT)
 METRIC = "G visibility-evals/s in MH-RE fit (batched vs plain), % of FP32 VALU peak"
 PEAK_FP32_TFLOPS = 157.3
 F_VIS, F_CPH, F_CHAIN = 170.0, 15.0, 40.0
+# quadratic system of 8 equations in 8 unknowns, monomial form: 36 products
+# + 8 * (45 FMA + 2) = 772 flop per chi^2 evaluation
+F_QUAD8 = 772.0
 
 
 def log(*a):
@@ -104,10 +114,12 @@ def batched_vs_plain(iters, nfits, log_every=16):
         fits.append((d, nb, ns))
         if (f + 1) % log_every == 0:
             log("  generated %d/%d fits" % (f + 1, nfits))
-    mk = lambda d, nb, ns, f: eht_fit(d, nb, ns, nburn=iters + 2, niter=0, seed=300 + f)
+    tune_iters = 2
+    mk = lambda d, nb, ns, f: eht_fit(d, nb, ns, nburn=iters + 2 + 6 * tune_iters, niter=0, seed=300 + f)
     total_vis = 0.0
     with Session([mk(d, nb, ns, f) for f, (d, nb, ns) in enumerate(fits)], persistent=1) as s:
         s.run(1)
+        wpb, pers, _ = s.autotune(tune_iters)
         s.sync()
         e0 = s.stats()
         t0 = time.time()
@@ -130,12 +142,96 @@ def batched_vs_plain(iters, nfits, log_every=16):
             tp += time.time() - t0
             plain_vis += s.stats()["vis_evals"] - e0["vis_evals"]
     plain = plain_vis / tp
-    return dict(fits=nfits, iterations=iters,
+    return dict(fits=nfits, iterations=iters, autotuned={"waves_per_block": wpb, "persistent": pers},
                 chains=int(sum(nb * ns for _, nb, ns in fits)),
                 vis_total=int(sum(d["nvis"] for d, _, _ in fits)),
                 batched_gvis_per_s=batched / 1e9, plain_sequential_gvis_per_s=plain / 1e9,
                 ratio=batched / plain,
                 pct_fp32_peak_batched=100.0 * batched * F_VIS / (PEAK_FP32_TFLOPS * 1e12))
+
+
+def adjoint_leg(npts=4096, nvis=20000, ncph=6000, reps=5, gn_fits=8):
+    """Config 4: batched chi^2 gradient throughput, then GN polish + Laplace."""
+    import torch
+    from cuda_mcmc_amd import adjoint, synthdata
+    from cuda_mcmc_amd.engine import Session
+    d = synthdata.ring_gauss_dataset(nvis=nvis, ncph=ncph, nsta=8, seed=4)
+    data = adjoint.EhtData.from_dataset(d)
+    rng = np.random.default_rng(7)
+    lo, hi = synthdata.EHT_PMIN.astype(np.float64), synthdata.EHT_PMAX.astype(np.float64)
+    P = torch.as_tensor((lo + (hi - lo) * rng.uniform(size=(npts, 9))).astype(np.float32), device="cuda")
+    data.chi2_grad(P)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        data.chi2_grad(P)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    e0.record()
+    for _ in range(reps):
+        data.chi2_grad(P, need_grad=False)
+    e1.record()
+    e1.synchronize()
+    ms_fwd = e0.elapsed_time(e1) / reps
+    out = dict(points=npts, nvis=nvis, ncph=ncph, ms_per_call=round(ms, 4), ms_chi2_only=round(ms_fwd, 4),
+               vis_grad_evals_per_s=npts * nvis / (ms * 1e-3),
+               model_evals_per_s=npts * (nvis + 3 * ncph) / (ms * 1e-3))
+    # short fits of the config-2 shape, then Gauss-Newton on each best chain
+    before, after, ratios = [], [], []
+    for f in range(gn_fits):
+        dd = synthdata.ring_gauss_dataset(nvis=2000, ncph=600, nsta=8, seed=500 + f)
+        fit = eht_fit(dd, 32, 128, nburn=600, niter=400, seed=500 + f)
+        with Session([fit]) as s:
+            s.run(fit.nburn + fit.niter)
+            s.download()
+        i = np.unravel_index(np.argmin(fit.chi2c), fit.chi2c.shape)
+        dat = adjoint.EhtData.from_dataset(dd)
+        p, c = adjoint.gauss_newton(dat, fit.pcur[i][None, :], synthdata.EHT_PMIN, synthdata.EHT_PMAX,
+                                    synthdata.EHT_PDESCR, iters=20)
+        before.append(float(fit.chi2c[i]))
+        after.append(float(c[0]))
+        cov, vv = adjoint.laplace_covariance(dat, p, synthdata.EHT_PDESCR)
+        smp = fit.pout[:, 0].reshape(9, -1).T.astype(np.float64)
+        smp[:, 8] = np.angle(np.exp(1j * (smp[:, 8] - p[0, 8]))) + p[0, 8]
+        sd_mc = smp[:, vv].std(axis=0)
+        ratios.append(float(np.median(sd_mc / np.sqrt(np.diag(cov[0])))))
+    out["gauss_newton"] = dict(fits=gn_fits, iterations=20, chi2_best_chain_median=float(np.median(before)),
+                               chi2_polished_median=float(np.median(after)),
+                               chi2_drop_median=float(np.median(np.array(before) - np.array(after))),
+                               laplace_vs_chain_sd_ratio_median=float(np.median(ratios)))
+    return out
+
+
+def quadsys_leg(nsys=1024, nburn=3000, niter=10000):
+    """Config 5: every quadratic system one fit of one session, then GN polish."""
+    from cuda_mcmc_amd import quadsys, synthdata
+    from cuda_mcmc_amd.engine import Session
+    ds = synthdata.quadsys_dataset(nsys=nsys, seed=5)
+    fits = [quadsys.quad_fit(ds["Q"][s], ds["b"][s], ds["c"][s], nbeta=16, nseq=32, nburn=nburn, niter=niter,
+                             seed=7000 + s, record=False) for s in range(nsys)]
+    with Session(fits) as s:
+        s.run(1)
+        s.sync()
+        e0 = s.stats()
+        t0 = time.time()
+        s.run(nburn + niter - 1)
+        s.sync()
+        dt = time.time() - t0
+        e1 = s.stats()
+        s.download()
+    ev = e1["chi2_evals"] - e0["chi2_evals"]
+    X0, c2 = quadsys.best_rows(fits)
+    t1 = time.time()
+    X, resid = quadsys.gauss_newton(ds["Q"], ds["b"], -ds["c"], X0, iters=20)
+    tgn = time.time() - t1
+    return dict(systems=nsys, chains=nsys * 16 * 32, iterations=nburn + niter - 1, seconds=round(dt, 3),
+                chi2_evals_per_s=ev / dt, tflops=ev * F_QUAD8 / dt / 1e12,
+                pct_fp32_peak=100.0 * ev * F_QUAD8 / dt / (PEAK_FP32_TFLOPS * 1e12),
+                gn_seconds=round(tgn, 3), solved=int((resid < 1e-5).sum()),
+                solved_fraction=float((resid < 1e-5).mean()),
+                best_chain_resid_median=float(np.median(np.sqrt(c2 / 1e4))))
 
 
 def main():
@@ -148,6 +244,8 @@ def main():
     ap.add_argument("--batched-fits", type=int, default=64)
     ap.add_argument("--batched-iters", type=int, default=4)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-quadsys", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: every fit and sequence is independent, so the job "
@@ -224,6 +322,12 @@ def main():
     if not args.no_batched:
         log("batched vs plain (%d fits)..." % args.batched_fits)
         out["batched"] = batched_vs_plain(args.batched_iters, args.batched_fits)
+    if not args.no_adjoint:
+        log("adjoint gradient + Gauss-Newton...")
+        out["adjoint"] = adjoint_leg()
+    if not args.no_quadsys:
+        log("quadratic systems (1024 x 8 equations)...")
+        out["quadsys"] = quadsys_leg()
     if not args.no_cpu_baseline:
         nth = args.cpu_threads or min(os.cpu_count() or 1, 16)
         log("cpu baseline on %d threads..." % nth)

@@ -122,6 +122,8 @@ def _declare(L):
     L.mhre_session_run.argtypes = [vp, C.c_int]
     L.mhre_session_time_kernels.restype = C.c_int
     L.mhre_session_time_kernels.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+    L.mhre_session_autotune.restype = C.c_int
+    L.mhre_session_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double)]
     L.mhre_session_sync.restype = C.c_int
     L.mhre_session_sync.argtypes = [vp]
     L.mhre_session_download.restype = C.c_int
@@ -140,6 +142,8 @@ def _declare(L):
     L.mhre_eht_gn_system.argtypes = [vp, vp, C.c_int, C.c_int, vp, vp, vp, vp]
     L.mhre_eht_data_destroy.restype = C.c_int
     L.mhre_eht_data_destroy.argtypes = [vp]
+    L.mhre_quad_jacobian.restype = C.c_int
+    L.mhre_quad_jacobian.argtypes = [fp, fp, fp, ip, C.c_int, C.c_int, C.c_int, fp, C.c_int, fp, fp]
 
 
 class EngineError(RuntimeError):

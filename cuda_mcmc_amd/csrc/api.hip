@@ -91,7 +91,7 @@ struct FitHost {
   MhreFitSpec s;
   int kind = K_USER;
   int nchain = 0;
-  int nvis_real = 0, ncph = 0, cpext = 0, nvis_k = 0, ntile = 0, ring_ok = 1;
+  int nvis_real = 0, ncph = 0, cpext = 0, nvis_k = 0, ntile = 0, ring_ok = 1, cgroup = 1;
   int quad_n = 0;
   // packed host inputs of the native kernels
   std::vector<float> vis;        // 4 per visibility
@@ -100,12 +100,14 @@ struct FitHost {
   std::vector<int> tinfo;        // 2 per tile: first closure, max u^2+v^2 (float bits)
   std::vector<float> qcoef;
   std::vector<int> sumh;
+  std::vector<uint32_t> hbits;   // packed acceptance history
   // arena offsets
   size_t o_vis, o_clo, o_clow, o_tinfo, o_qcoef;
   size_t o_coor, o_dat, o_std2r, o_icoor, o_idat, o_pdescr, o_ivar, o_invar, o_pmint, o_pmaxt, o_beta;
   size_t o_pcur, o_chi2c, o_pstp, o_ncnt, o_nacpt, o_nhist, o_nexch, o_sumh, o_tcur, o_flag, o_ptent, o_ptentn;
   size_t o_pout, o_tout, o_chi2, o_datm, o_chi2m, o_mc;
   bool has_datm = false;
+  bool record = true;            // pout/tout/chi2 given
   FitDesc d{};
 };
 
@@ -120,7 +122,8 @@ int validate(const MhreFitSpec& s, int idx) {
       !s.pstp || !s.n_cnt || !s.n_acpt || !s.n_hist || !s.n_exch || !s.tcur || !s.flag || !s.ptent ||
       !s.ptentn || !s.dat || !s.std2r)
     FAIL(MHRE_EINVAL, "fit %d: a required array pointer is NULL", idx);
-  if (s.niter > 0 && (!s.pout || !s.tout || !s.chi2)) FAIL(MHRE_EINVAL, "fit %d: pout/tout/chi2 NULL", idx);
+  if (s.niter > 0 && (!s.pout || !s.tout || !s.chi2) && (s.pout || s.tout || s.chi2))
+    FAIL(MHRE_EINVAL, "fit %d: pout, tout and chi2 must be all given or all NULL", idx);
   int nv = 0;
   for (int i = 0; i < s.nptot; ++i) {
     if (s.pdescr[i] != 0) {
@@ -165,7 +168,9 @@ int pack_eht(FitHost& h, int idx) {
   h.ncph = ncph;
   h.cpext = cpext;
   h.nvis_k = nvis + (cpext ? 3 * ncph : 0);
-  h.vis.assign((size_t)h.nvis_k * 4, 0.0f);
+  h.ntile = (h.nvis_k + TILE - 1) / TILE;
+  // one extra tile of zero-weight padding: the kernel prefetches tile t + 1
+  h.vis.assign((size_t)(h.ntile + 1) * TILE * 4, 0.0f);
   for (int i = 0; i < nvis; ++i) {
     h.vis[4 * i + 0] = s.coor[2 * i];
     h.vis[4 * i + 1] = s.coor[2 * i + 1];
@@ -192,23 +197,34 @@ int pack_eht(FitHost& h, int idx) {
       }
     }
   }
-  h.ntile = (h.nvis_k + TILE - 1) / TILE;
-  // closures are handled after the tile holding their largest index, as long
-  // as all three indices are still in the RING-deep LDS window
-  std::vector<int> tile_of(ncph);
-  bool ok = true;
+  // Closures are formed from the LDS ring after every cgroup-th tile, as long
+  // as all three indices are still in the RING-deep window at that point.
+  // cgroup aims at about one wave of closures per pass.
+  std::vector<int> tile_of(ncph), lo_of(ncph);
   for (int j = 0; j < ncph; ++j) {
     const int a = tri[3 * j], b = tri[3 * j + 1], c = tri[3 * j + 2];
-    const int mx = std::max(a, std::max(b, c)), mn = std::min(a, std::min(b, c));
-    tile_of[j] = mx / TILE;
-    if (mn < (tile_of[j] + 1) * TILE - RING) ok = false;
+    tile_of[j] = std::max(a, std::max(b, c)) / TILE;
+    lo_of[j] = std::min(a, std::min(b, c));
   }
+  auto window_ok = [&](int g) {
+    for (int j = 0; j < ncph; ++j) {
+      const int tp = std::min((tile_of[j] / g + 1) * g - 1, h.ntile - 1);   // tile after which it is formed
+      if (lo_of[j] < (tp + 1) * TILE - RING) return false;
+    }
+    return true;
+  };
+  int g = ncph > 0 ? (int)std::lround((double)TILE * h.ntile / ncph) : 1;
+  g = std::max(1, std::min(MAX_CGROUP, g));
+  while (g > 1 && !window_ok(g)) --g;
+  const bool ok = window_ok(g);
+  h.cgroup = g;
   h.ring_ok = ok ? 1 : 0;
   std::vector<int> order(ncph);
   for (int j = 0; j < ncph; ++j) order[j] = j;
   if (ok) std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return tile_of[x] < tile_of[y]; });
-  h.clo.assign((size_t)ncph * 4, 0);
-  h.clow.assign((size_t)ncph * 4, 0.0f);
+  // one wave of zero padding: lanes prefetch closure c0 + lane
+  h.clo.assign(((size_t)ncph + TILE) * 4, 0);
+  h.clow.assign(((size_t)ncph + TILE) * 4, 0.0f);
   for (int q = 0; q < ncph; ++q) {
     const int j = order[q];
     h.clo[4 * q + 0] = tri[3 * j];
@@ -220,10 +236,10 @@ int pack_eht(FitHost& h, int idx) {
     h.clow[4 * q + 1] = (float)sin(obs);
     h.clow[4 * q + 2] = s.std2r[nvis + j];
   }
-  h.tinfo.assign(2 * ((size_t)h.ntile + 1), 0);
+  h.tinfo.assign(2 * ((size_t)h.ntile + 2), 0);
   if (ok) {
     int q = 0;
-    for (int t = 0; t <= h.ntile; ++t) {
+    for (int t = 0; t <= h.ntile + 1; ++t) {
       while (q < ncph && tile_of[order[q]] < t) ++q;
       h.tinfo[2 * t] = q;
     }
@@ -264,12 +280,66 @@ int pack_quad(FitHost& h, int idx) {
   return MHRE_OK;
 }
 
+// ------------------------------------------------------- XCD work ranges
+
+// Re-orders items (chunks of up to XCHUNK consecutive items of one fit) into
+// NXCD contiguous ranges of nearly equal cost: chunks, in their given
+// (heaviest first) order, go to the range with the least cost so far.
+XcdRanges deal_xcd(std::vector<WorkItem>& items, const std::vector<double>& fit_cost) {
+  struct Chunk { size_t b, e; };
+  std::vector<Chunk> chunks;
+  for (size_t i = 0; i < items.size();) {
+    size_t j = i;
+    while (j < items.size() && j - i < (size_t)XCHUNK && items[j].fit == items[i].fit) ++j;
+    chunks.push_back({i, j});
+    i = j;
+  }
+  std::vector<std::vector<size_t>> lists(NXCD);
+  double load[NXCD] = {};
+  for (size_t c = 0; c < chunks.size(); ++c) {
+    int x = 0;
+    for (int y = 1; y < NXCD; ++y)
+      if (load[y] < load[x]) x = y;
+    lists[x].push_back(c);
+    load[x] += (double)(chunks[c].e - chunks[c].b) * fit_cost[items[chunks[c].b].fit];
+  }
+  std::vector<WorkItem> out;
+  out.reserve(items.size());
+  XcdRanges r{};
+  for (int x = 0; x < NXCD; ++x) {
+    r.off[x] = (int)out.size();
+    for (size_t c : lists[x]) out.insert(out.end(), items.begin() + chunks[c].b, items.begin() + chunks[c].e);
+  }
+  r.off[NXCD] = (int)out.size();
+  items.swap(out);
+  return r;
+}
+
+// ------------------------------------------------- acceptance history bits
+
+int hist_words(int nadj) { return (nadj + 31) / 32; }
+
+// n_hist[a][q] (0/1, q = k * nchain + chain) -> bit a % 32 of word [a / 32][q]
+void pack_hist(const int* n_hist, int nadj, size_t npc, std::vector<uint32_t>& bits) {
+  bits.assign((size_t)hist_words(nadj) * npc, 0u);
+  for (int a = 0; a < nadj; ++a)
+    for (size_t q = 0; q < npc; ++q)
+      if (n_hist[(size_t)a * npc + q]) bits[(size_t)(a >> 5) * npc + q] |= 1u << (a & 31);
+}
+
+void unpack_hist(const std::vector<uint32_t>& bits, int nadj, size_t npc, int* n_hist) {
+  for (int a = 0; a < nadj; ++a)
+    for (size_t q = 0; q < npc; ++q)
+      n_hist[(size_t)a * npc + q] = (int)((bits[(size_t)(a >> 5) * npc + q] >> (a & 31)) & 1u);
+}
+
 // ------------------------------------------------------------- session
 
 struct KindWork {
   int kind;
   int quad_n;
   std::vector<WorkItem> step, eval;
+  XcdRanges xr{};   // wave-per-chain kinds: step list dealt into per-XCD ranges
   WorkItem* d_step = nullptr;
   WorkItem* d_eval = nullptr;
   int qslot = 0;
@@ -288,9 +358,10 @@ struct Session {
   int n_init = 0;
   WorkItem* d_x = nullptr;
   int n_x = 0;
+  XcdRanges xr_x{};   // exchange list per XCD
   std::vector<KindWork> kinds;
   MhreOptions opt{};
-  int wpb = 4, persistent = 0, grid = 0;
+  int wpb = 4, persistent = 0, grid = 0, ncu = 1;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int graph_iters = 0;
@@ -370,6 +441,7 @@ StepLaunch make_launch(Session* S, const KindWork& k) {
   L.fits = S->d_fits;
   L.work = k.d_step;
   L.nwork = (int)k.step.size();
+  L.xr = k.xr;
   L.clk = S->d_clk;
   L.wpb = S->wpb;
   L.persistent = S->persistent;
@@ -399,7 +471,7 @@ int launch_iteration(Session* S) {
   L.tr_ex = S->d_tr_ex;
   L.tr_it0 = S->tr_it0;
   L.tr_nit = S->tr_nit;
-  HIPCHK(launch_exchange(L, S->d_x, S->n_x, S->stream));
+  HIPCHK(launch_exchange(L, S->d_x, S->xr_x, S->stream));
   return MHRE_OK;
 }
 
@@ -426,7 +498,7 @@ int time_kernels(Session* S, int niter, double* step_ms, double* exch_ms) {
     StepLaunch L{};
     L.fits = S->d_fits;
     L.clk = S->d_clk;
-    HIPCHK(launch_exchange(L, S->d_x, S->n_x, S->stream));
+    HIPCHK(launch_exchange(L, S->d_x, S->xr_x, S->stream));
     HIPCHK(hipEventRecord(e[2], S->stream));
     HIPCHK(hipEventSynchronize(e[2]));
     float a = 0.0f, b = 0.0f;
@@ -440,6 +512,45 @@ int time_kernels(Session* S, int niter, double* step_ms, double* exch_ms) {
   if (step_ms) *step_ms = niter > 0 ? st / niter : 0.0;
   if (exch_ms) *exch_ms = niter > 0 ? ex / niter : 0.0;
   return rc;
+}
+
+void drop_graph(Session* S) {
+  if (S->gexec) hipGraphExecDestroy(S->gexec);
+  if (S->graph) hipGraphDestroy(S->graph);
+  S->gexec = nullptr;
+  S->graph = nullptr;
+}
+
+// Picks the launch form of the wave-per-chain step kernels by timing real
+// iterations (they advance the chains like any other iteration): each
+// (waves per work-group, grid or persistent) candidate runs `iters`
+// iterations under HIP events and the fastest stays.  All forms give
+// bit-identical chains.  A captured graph is dropped and re-captured later.
+int autotune(Session* S, int iters, int* best_wpb, int* best_persistent, double* best_ms) {
+  static const int cand[6][2] = {{4, 0}, {2, 0}, {1, 0}, {4, 1}, {2, 1}, {1, 1}};
+  if (iters <= 0) FAIL(MHRE_EINVAL, "autotune: iters must be positive");
+  double best = 1e300;
+  int bw = S->wpb, bp = S->persistent;
+  for (const auto& c : cand) {
+    S->wpb = c[0];
+    S->persistent = c[1];
+    S->grid = std::max(1, S->ncu * (16 / S->wpb));
+    double st = 0.0, ex = 0.0;
+    CHK(time_kernels(S, iters, &st, &ex));
+    if (st < best) {
+      best = st;
+      bw = c[0];
+      bp = c[1];
+    }
+  }
+  S->wpb = bw;
+  S->persistent = bp;
+  S->grid = std::max(1, S->ncu * (16 / S->wpb));
+  drop_graph(S);
+  if (best_wpb) *best_wpb = bw;
+  if (best_persistent) *best_persistent = bp;
+  if (best_ms) *best_ms = best;
+  return MHRE_OK;
 }
 
 int launch_eval_all(Session* S, int outputs, int respect_flag) {
@@ -512,16 +623,18 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
     h.o_pstp = A.reserve<float>(P * C);
     h.o_ncnt = A.reserve<int>(P * C);
     h.o_nacpt = A.reserve<int>(P * C);
-    h.o_nhist = A.reserve<int>((size_t)s.nadj * P * C);
+    h.o_nhist = A.reserve<uint32_t>((size_t)hist_words(s.nadj) * P * C);
     h.o_nexch = A.reserve<int>(C);
     h.o_sumh = A.reserve<int>(P * C);
     h.o_tcur = A.reserve<int>(C);
     h.o_flag = A.reserve<int>(C);
     h.o_ptent = A.reserve<float>(C);
     h.o_ptentn = A.reserve<int>(C);
-    h.o_pout = A.reserve<float>(P * C * (size_t)s.niter);
-    h.o_tout = A.reserve<int>(C * (size_t)s.niter);
-    h.o_chi2 = A.reserve<float>((size_t)s.nseq * s.niter);
+    h.record = s.pout != nullptr;
+    const size_t nrec = h.record ? (size_t)s.niter : 0;
+    h.o_pout = A.reserve<float>(P * C * nrec);
+    h.o_tout = A.reserve<int>(C * nrec);
+    h.o_chi2 = A.reserve<float>((size_t)s.nseq * nrec);
     h.has_datm = h.kind == K_USER || s.datm != nullptr;
     const size_t ndatm = h.has_datm ? std::max<size_t>((size_t)std::max(s.ndatm, 0), C * s.ndat) : 0;
     h.o_datm = A.reserve<float>(ndatm);
@@ -558,12 +671,13 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
     d.imodel = s.imodel;
     d.nbeta = s.nbeta; d.nseq = s.nseq; d.nptot = s.nptot; d.nprm = s.nprm; d.nadj = s.nadj; d.npass = s.npass;
     d.ndat = s.ndat;
-    d.nvis = h.nvis_k; d.ncph = h.ncph; d.ntile = h.ntile; d.ring_ok = h.ring_ok;
+    d.nvis = h.nvis_k; d.ncph = h.ncph; d.ntile = h.ntile; d.ring_ok = h.ring_ok; d.cgroup = h.cgroup;
     d.nburn = s.nburn; d.niter = s.niter; d.nchain = h.nchain;
     d.key0 = (uint32_t)(s.seed & 0xffffffffull);
     d.key1 = (uint32_t)(s.seed >> 32);
     d.adapt_target = s.adapt_target > 0.0f ? s.adapt_target : 0.30f;
     d.quad_n = h.quad_n; d.quad_neq = s.ndat;
+    d.record = h.record ? 1 : 0;
     d.vis = at<float4>(S, h.o_vis); d.clo = at<int4>(S, h.o_clo); d.clow = at<float4>(S, h.o_clow);
     d.tinfo = at<int2>(S, h.o_tinfo); d.qcoef = at<float>(S, h.o_qcoef);
     d.coor = at<float>(S, h.o_coor); d.dat = at<float>(S, h.o_dat); d.std2r = at<float>(S, h.o_std2r);
@@ -571,13 +685,13 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
     d.pdescr = at<int>(S, h.o_pdescr); d.ivar = at<int>(S, h.o_ivar); d.invar = at<int>(S, h.o_invar);
     d.pmint = at<float>(S, h.o_pmint); d.pmaxt = at<float>(S, h.o_pmaxt); d.beta = at<float>(S, h.o_beta);
     d.pcur = at<float>(S, h.o_pcur); d.chi2c = at<float>(S, h.o_chi2c); d.pstp = at<float>(S, h.o_pstp);
-    d.n_cnt = at<int>(S, h.o_ncnt); d.n_acpt = at<int>(S, h.o_nacpt); d.n_hist = at<int>(S, h.o_nhist);
+    d.n_cnt = at<int>(S, h.o_ncnt); d.n_acpt = at<int>(S, h.o_nacpt); d.hbits = at<uint32_t>(S, h.o_nhist);
+    d.hwords = hist_words(s.nadj);
     d.n_exch = at<int>(S, h.o_nexch); d.sumh = at<int>(S, h.o_sumh); d.tcur = at<int>(S, h.o_tcur);
     d.flag = at<int>(S, h.o_flag); d.ptent = at<float>(S, h.o_ptent); d.ptentn = at<int>(S, h.o_ptentn);
     d.pout = at<float>(S, h.o_pout); d.tout = at<int>(S, h.o_tout); d.chi2 = at<float>(S, h.o_chi2);
     d.datm = h.has_datm ? at<float>(S, h.o_datm) : nullptr;
     d.chi2m = h.has_datm ? at<float>(S, h.o_chi2m) : nullptr;
-    d.vscratch = nullptr;
     d.mc = at<CCalcModel>(S, h.o_mc);
     d.nevals = S->d_nevals + 2 * i;
     d.errflag = S->d_err;
@@ -604,7 +718,8 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
     CHK(h2d(S, h.o_pstp, s.pstp, P * C));
     CHK(h2d(S, h.o_ncnt, s.n_cnt, P * C));
     CHK(h2d(S, h.o_nacpt, s.n_acpt, P * C));
-    CHK(h2d(S, h.o_nhist, s.n_hist, (size_t)s.nadj * P * C));
+    pack_hist(s.n_hist, s.nadj, P * C, h.hbits);
+    CHK(h2d(S, h.o_nhist, h.hbits.data(), h.hbits.size()));
     CHK(h2d(S, h.o_nexch, s.n_exch, C));
     CHK(h2d(S, h.o_tcur, s.tcur, C));
     CHK(h2d(S, h.o_flag, s.flag, C));
@@ -634,6 +749,9 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
   std::vector<WorkItem> init, xw;
   const int kinds_order[4] = {K_SGRA9, K_SGRA13, K_QUAD, K_USER};
   int qslot = 0;
+  // per-chain step cost ~ data terms evaluated per chi^2
+  std::vector<double> fit_cost(nfits);
+  for (int i = 0; i < nfits; ++i) fit_cost[i] = (double)S->fits[i].nvis_k + S->fits[i].s.ndat + 1.0;
   for (int kk : kinds_order) {
     std::vector<int> idx;
     for (int i = 0; i < nfits; ++i)
@@ -664,13 +782,24 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
     kw.qslot = qslot++;
     for (int i : idx)
       for (int c = 0; c < S->fits[i].nchain; ++c) { kw.step.push_back({i, c}); kw.eval.push_back({i, c}); }
+    kw.xr = deal_xcd(kw.step, fit_cost);
     S->kinds.push_back(std::move(kw));
   }
+  std::vector<double> seq_cost(nfits);
   for (int i = 0; i < nfits; ++i) {
     const FitHost& h = S->fits[i];
+    seq_cost[i] = (double)h.s.nbeta;
     if (h.s.init_chains)
       for (int c = 0; c < h.nchain; ++c) init.push_back({i, c});
-    for (int q = 0; q < h.s.nseq; ++q) xw.push_back({i, q});
+  }
+  {
+    // exchange list, heaviest fits first, dealt like the step lists
+    std::vector<int> order(nfits);
+    for (int i = 0; i < nfits; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return seq_cost[a] > seq_cost[b]; });
+    for (int i : order)
+      for (int q = 0; q < S->fits[i].s.nseq; ++q) xw.push_back({i, q});
+    S->xr_x = deal_xcd(xw, seq_cost);
   }
   for (auto& k : S->kinds) {
     CHK(dev_alloc(S, &k.d_step, k.step.size()));
@@ -690,6 +819,7 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
   int dev = 0, ncu = 0;
   HIPCHK(hipGetDevice(&dev));
   HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  S->ncu = ncu;
   S->grid = std::max(1, ncu * (16 / S->wpb));
 
   // decision trace (single fit)
@@ -775,17 +905,22 @@ int download(Session* S, int model_data) {
     CHK(d2h(S, s.pstp, h.o_pstp, P * C));
     CHK(d2h(S, s.n_cnt, h.o_ncnt, P * C));
     CHK(d2h(S, s.n_acpt, h.o_nacpt, P * C));
-    CHK(d2h(S, s.n_hist, h.o_nhist, (size_t)s.nadj * P * C));
+    h.hbits.resize((size_t)hist_words(s.nadj) * P * C);
+    CHK(d2h(S, h.hbits.data(), h.o_nhist, h.hbits.size()));
     CHK(d2h(S, s.n_exch, h.o_nexch, C));
     CHK(d2h(S, s.tcur, h.o_tcur, C));
     CHK(d2h(S, s.flag, h.o_flag, C));
     CHK(d2h(S, s.ptent, h.o_ptent, C));
     CHK(d2h(S, s.ptentn, h.o_ptentn, C));
-    CHK(d2h(S, s.pout, h.o_pout, P * C * (size_t)s.niter));
-    CHK(d2h(S, s.tout, h.o_tout, C * (size_t)s.niter));
-    CHK(d2h(S, s.chi2, h.o_chi2, (size_t)s.nseq * s.niter));
+    if (h.record) {
+      CHK(d2h(S, s.pout, h.o_pout, P * C * (size_t)s.niter));
+      CHK(d2h(S, s.tout, h.o_tout, C * (size_t)s.niter));
+      CHK(d2h(S, s.chi2, h.o_chi2, (size_t)s.nseq * s.niter));
+    }
   }
   HIPCHK(hipStreamSynchronize(S->stream));
+  for (FitHost& h : S->fits)
+    unpack_hist(h.hbits, h.s.nadj, (size_t)h.s.nprm * h.nchain, h.s.n_hist);
   if (model_data) {
     // model data and chi^2 terms of each chain's final state; chi2c is
     // recomputed identically and already copied out above
@@ -920,7 +1055,7 @@ int eht_data_create(const float* coor, const float* dat, const float* std2r, con
   d.imodel = imodel;
   d.nptot = nptot;
   d.ndat = ndat;
-  d.nvis = h.nvis_k; d.ncph = h.ncph; d.ntile = h.ntile; d.ring_ok = h.ring_ok;
+  d.nvis = h.nvis_k; d.ncph = h.ncph; d.ntile = h.ntile; d.ring_ok = h.ring_ok; d.cgroup = h.cgroup;
   d.vis = reinterpret_cast<const float4*>(E->dev + o_vis);
   d.clo = reinterpret_cast<const int4*>(E->dev + o_clo);
   d.clow = reinterpret_cast<const float4*>(E->dev + o_clow);
@@ -979,6 +1114,12 @@ int mhre_session_run(void* session, int niter) {
 int mhre_session_time_kernels(void* session, int niter, double* step_ms, double* exch_ms) {
   if (!session) FAIL(MHRE_EINVAL, "NULL session");
   return time_kernels(static_cast<Session*>(session), niter, step_ms, exch_ms);
+}
+
+int mhre_session_autotune(void* session, int iters, int* waves_per_block, int* persistent, double* step_ms) {
+  if (!session) FAIL(MHRE_EINVAL, "null session");
+  (void)hipGetLastError();
+  return autotune(static_cast<Session*>(session), iters, waves_per_block, persistent, step_ms);
 }
 
 int mhre_session_sync(void* session) {
@@ -1167,6 +1308,42 @@ int mhre_eht_gn_system(void* handle, const float* params, int n, int ldp, double
 
 int mhre_eht_data_destroy(void* handle) {
   destroy_eht_data(static_cast<EhtData*>(handle));
+  return MHRE_OK;
+}
+
+int mhre_quad_jacobian(const float* Q, const float* b, const float* dat, const int* sys, int nsys, int neq, int n,
+                       const float* X, int rows, float* r, float* J) {
+  (void)hipGetLastError();
+  if (nsys <= 0 || neq <= 0 || n <= 0 || n > MAXP || rows < 0) FAIL(MHRE_EINVAL, "mhre_quad_jacobian: bad sizes");
+  if (!Q || !b || !dat || !X || !r || !J) FAIL(MHRE_EINVAL, "mhre_quad_jacobian: null array");
+  if (!sys && rows > nsys) FAIL(MHRE_EINVAL, "mhre_quad_jacobian: rows %d > nsys %d without a row map", rows, nsys);
+  if (sys)
+    for (int i = 0; i < rows; ++i)
+      if (sys[i] < 0 || sys[i] >= nsys) FAIL(MHRE_EINVAL, "mhre_quad_jacobian: sys[%d] = %d out of range", i, sys[i]);
+  if (rows == 0) return MHRE_OK;
+  const size_t nq = (size_t)nsys * neq * n * n, nb = (size_t)nsys * neq * n, nd = (size_t)nsys * neq;
+  const size_t nx = (size_t)rows * n, nr = (size_t)rows * neq, nj = nr * n;
+  Arena A;
+  const size_t oQ = A.reserve<float>(nq), ob = A.reserve<float>(nb), od = A.reserve<float>(nd);
+  const size_t os = A.reserve<int>(sys ? rows : 0), oX = A.reserve<float>(nx);
+  const size_t orr = A.reserve<float>(nr), oJ = A.reserve<float>(nj);
+  char* dev = nullptr;
+  HIPCHK(hipMalloc((void**)&dev, A.bytes));
+  struct Free {
+    char* p;
+    ~Free() { hipFree(p); }
+  } guard{dev};
+  HIPCHK(hipMemcpy(dev + oQ, Q, nq * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dev + ob, b, nb * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dev + od, dat, nd * sizeof(float), hipMemcpyHostToDevice));
+  if (sys) HIPCHK(hipMemcpy(dev + os, sys, rows * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dev + oX, X, nx * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(launch_quad_jac(reinterpret_cast<float*>(dev + oQ), reinterpret_cast<float*>(dev + ob),
+                         reinterpret_cast<float*>(dev + od), sys ? reinterpret_cast<int*>(dev + os) : nullptr,
+                         reinterpret_cast<float*>(dev + oX), rows, neq, n, reinterpret_cast<float*>(dev + orr),
+                         reinterpret_cast<float*>(dev + oJ), nullptr));
+  HIPCHK(hipMemcpy(r, dev + orr, nr * sizeof(float), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(J, dev + oJ, nj * sizeof(float), hipMemcpyDeviceToHost));
   return MHRE_OK;
 }
 

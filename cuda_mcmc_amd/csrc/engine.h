@@ -5,10 +5,19 @@
 
 namespace mhre {
 
+// Optional record of the decisions of iterations it0 .. it0+nit-1.
+struct TraceBuf {
+  int* dec;      // [nit][nprm][nchain] : k | inprior << 8 | accept << 9
+  float* chi2;   // [nit][nprm][nchain]
+  int* ex;       // [nit][nbeta][nseq] : k | accept << 16
+  int it0, nit;
+};
+
 struct StepLaunch {
   const FitDesc* fits;   // device
   const WorkItem* work;  // device, step work list
   int nwork;
+  XcdRanges xr;          // wave-per-chain kernels: per-XCD ranges of work
   RunClock* clk;         // device
   int wpb;               // waves per work-group (1, 2, 4)
   int persistent;        // 1: grid-stride work queue
@@ -23,7 +32,7 @@ hipError_t launch_eval_eht(int model, const FitDesc* fits, const WorkItem* work,
 hipError_t launch_step_eht(int model, const StepLaunch& L, hipStream_t s);
 hipError_t launch_eval_quad(int n, const FitDesc* fits, const WorkItem* work, int nwork, hipStream_t s);
 hipError_t launch_step_quad(int n, const StepLaunch& L, hipStream_t s);
-hipError_t launch_exchange(const StepLaunch& L, const WorkItem* xwork, int nx, hipStream_t s);
+hipError_t launch_exchange(const StepLaunch& L, const WorkItem* xwork, const XcdRanges& xr, hipStream_t s);
 
 // generic.hip: the library's compiled-in model.cuh (reference contract)
 hipError_t launch_eval_user(const FitDesc* fits, const WorkItem* work, int nwork, int model_only,

@@ -83,6 +83,14 @@ __global__ void k_init_chains(const FitDesc* fits, const WorkItem* work, int nwo
 template <class T>
 __device__ inline GLOBAL T* gptr(T* p) { return (GLOBAL T*)p; }
 
+// Element idx of a global array through a 32-bit byte offset from a
+// wave-uniform base (one global_load with a scalar base, no 64-bit address
+// arithmetic per lane).
+template <class T>
+__device__ inline T ldg(const GLOBAL T* base, int idx) {
+  return *(const GLOBAL T*)((const GLOBAL char*)base + (uint32_t)idx * (uint32_t)sizeof(T));
+}
+
 struct EhtView {
   const GLOBAL float4* vis;
   const GLOBAL int4* clo;
@@ -91,14 +99,14 @@ struct EhtView {
   GLOBAL float* datm;
   GLOBAL float* chi2m;
   long long ndatm;
-  int nvis, ncph, ntile, ring_ok, ndat, nchain;
+  int nvis, ncph, ntile, ring_ok, cgroup, ndat, nchain;
 };
 
 __device__ inline EhtView eht_view(const FitDesc& f) {
   EhtView v;
   v.vis = gptr(f.vis); v.clo = gptr(f.clo); v.clow = gptr(f.clow); v.tinfo = gptr(f.tinfo);
   v.datm = gptr(f.datm); v.chi2m = gptr(f.chi2m); v.ndatm = f.ndatm;
-  v.nvis = f.nvis; v.ncph = f.ncph; v.ntile = f.ntile; v.ring_ok = f.ring_ok;
+  v.nvis = f.nvis; v.ncph = f.ncph; v.ntile = f.ntile; v.ring_ok = f.ring_ok; v.cgroup = f.cgroup;
   v.ndat = f.ndat; v.nchain = f.nchain;
   return v;
 }
@@ -125,9 +133,10 @@ __device__ inline float closure_term(float2 a, float2 b, float2 e, float4 ow, fl
 }
 
 // chi^2 of one chain whose model invariants are q.  Lanes stream 64
-// visibilities per step (the next tile's data and this tile's closure inputs
+// visibilities per step (the next tile's data and the pending closures' inputs
 // are fetched before the model is evaluated); model visibilities go to a
-// per-wave LDS ring from which the closures of the tile are formed.
+// per-wave LDS ring, from which the closures are formed after every cgroup-th
+// tile, so that one pass over the closures keeps most lanes busy.
 // The fp32 terms are accumulated in fp64 per lane and summed across the wave.
 // OUT: also write datm/chi2m of chain ch in the reference layout.
 template <class M, bool OUT>
@@ -136,73 +145,73 @@ __device__ double eht_chi2_wave(const EhtView& v, const typename M::Inv& q, floa
   const bool clos = v.ncph > 0 && v.ring_ok;
   const bool out = OUT && v.datm != nullptr;
   const int nvr = v.ndat - v.ncph;   // visibilities with an amplitude datum
-  const int lastv = v.nvis - 1;
-  const int lastc = v.ncph > 0 ? v.ncph - 1 : 0;
   double acc = 0.0;
-  // All loads below are unconditional with clamped (in-bounds) indices so the
-  // compiler can keep them in flight across the model evaluation.
-  float4 dn = v.vis[min(lane, lastv)];
-  // per tile: (first closure after it, max rho^2); carried one tile ahead
-  int c0 = 0;
-  int2 ti = v.tinfo[min(1, v.ntile)];
-  float r2cur = __int_as_float(v.tinfo[0].y);
+  // The packed arrays are padded (vis by one tile of zero weight, closures by
+  // one wave, tinfo by two entries), so every load below is in bounds and the
+  // loads can stay in flight across the model evaluation.
+  float4 dn = ldg(v.vis, lane);
+  int c0 = 0;           // first closure not yet formed
+  int kleft = v.cgroup;
+  int2 ti = ldg(v.tinfo, 1);
+  float r2cur = __int_as_float(ldg(v.tinfo, 0).y);
   for (int t = 0; t < v.ntile; ++t) {
     const int i = t * TILE + lane;
     const float4 d = dn;
-    dn = v.vis[min(i + TILE, lastv)];
-    const int2 tin = v.tinfo[min(t + 2, v.ntile)];
-    const int c1 = clos ? ti.x : 0;
+    dn = ldg(v.vis, i + TILE);
+    const int2 tin = ldg(v.tinfo, t + 2);
     const int cf = c0 + lane;
-    // (clo, clow and tinfo are allocated, at least one element, for every
-    // native fit, so these loads are always in bounds)
-    int4 ix = v.clo[min(cf, lastc)];
-    float4 ow = v.clow[min(cf, lastc)];
-    const bool live = i < v.nvis;
+    int4 ix = ldg(v.clo, cf);
+    float4 ow = ldg(v.clow, cf);
     const bool any_large = q.r.se * r2cur > 64.0f;
     const float2 V = M::vis(q, d.x, d.y, any_large);
     const float amp = __builtin_amdgcn_sqrtf(fmaf(V.x, V.x, V.y * V.y));
     const float r = d.z - amp;
-    const float term = r * r * d.w;
-    if (live) acc += (double)term;
-    if (out && live && i < nvr) {
+    acc += (double)(r * r * d.w);   // zero weight past the amplitude data
+    if (out && i < nvr) {
       const size_t imd = (size_t)ch * v.ndat + i;
       v.datm[imd] = amp;
-      v.chi2m[imd] = term;
+      v.chi2m[imd] = r * r * d.w;
       const long long tail = (long long)v.nchain * v.ndat + (long long)ch * nvr + i;
       if (tail < v.ndatm) v.datm[tail] = atan2_fast(V.y, V.x);
     }
-    // Lanes past nvis write slots of indices four tiles back, which no closure
-    // of this tile reads (closures span at most RING - TILE indices).
-    if (clos) ring[i & (RING - 1)] = V;
-    if (clos && c1 > c0) {
-      ring_fence();
-      for (int c = cf; c < c1; c += TILE) {
-        if (c != cf) {
-          ix = v.clo[c];
-          ow = v.clow[c];
-        }
-        float mc;
-        const float cterm = closure_term(ring[ix.x & (RING - 1)], ring[ix.y & (RING - 1)],
-                                         ring[ix.z & (RING - 1)], ow, out ? &mc : nullptr);
-        acc += (double)cterm;
-        if (out) {
-          const size_t imd = (size_t)ch * v.ndat + ix.w;
-          v.datm[imd] = mc;
-          v.chi2m[imd] = cterm;
+    if (clos) {
+      // Lanes past nvis write slots of indices RING back, which no pending
+      // closure reads (the host checked every closure's window).
+      ring[i & (RING - 1)] = V;
+      const int c1 = ti.x;   // closures whose largest index is in tiles <= t
+      if (--kleft == 0 || t + 1 == v.ntile) {
+        kleft = v.cgroup;
+        if (c1 > c0) {
+          ring_fence();
+          for (int c = cf; c < c1; c += TILE) {
+            if (c != cf) {
+              ix = ldg(v.clo, c);
+              ow = ldg(v.clow, c);
+            }
+            float mc;
+            const float cterm = closure_term(ring[ix.x & (RING - 1)], ring[ix.y & (RING - 1)],
+                                             ring[ix.z & (RING - 1)], ow, out ? &mc : nullptr);
+            acc += (double)cterm;
+            if (out) {
+              const size_t imd = (size_t)ch * v.ndat + ix.w;
+              v.datm[imd] = mc;
+              v.chi2m[imd] = cterm;
+            }
+          }
+          ring_fence();
+          c0 = c1;
         }
       }
-      ring_fence();
     }
-    c0 = c1;
     r2cur = __int_as_float(ti.y);
     ti = tin;
   }
   if (v.ncph && !v.ring_ok) {
     // General closure layout: recompute the three model visibilities.
     for (int c = lane; c < v.ncph; c += TILE) {
-      const int4 ix = v.clo[c];
-      const float4 ow = v.clow[c];
-      const float4 da = v.vis[ix.x], db = v.vis[ix.y], de = v.vis[ix.z];
+      const int4 ix = ldg(v.clo, c);
+      const float4 ow = ldg(v.clow, c);
+      const float4 da = ldg(v.vis, ix.x), db = ldg(v.vis, ix.y), de = ldg(v.vis, ix.z);
       float mc;
       const float term = closure_term(M::vis(q, da.x, da.y, true), M::vis(q, db.x, db.y, true),
                                       M::vis(q, de.x, de.y, true), ow, out ? &mc : nullptr);
@@ -257,13 +266,6 @@ __global__ void __launch_bounds__(256) k_eval_eht(const FitDesc* fits, const Wor
 
 // ------------------------------------------------------------ step kernel
 
-struct TraceBuf {
-  int* dec;      // [nit][nprm][nchain] : k | inprior << 8 | accept << 9
-  float* chi2;   // [nit][nprm][nchain]
-  int* ex;       // [nit][nbeta][nseq] : k | accept << 16
-  int it0, nit;
-};
-
 template <class M>
 __device__ void step_one_chain(const FitDesc& f, int ch, int it, float2* ring, const TraceBuf& tr) {
   const int lane = lane_id();
@@ -281,6 +283,13 @@ __device__ void step_one_chain(const FitDesc& f, int ch, int it, float2* ring, c
   int cnt = lane < P ? f.n_cnt[(size_t)lane * C + ch] : 0;
   int nac = lane < P ? f.n_acpt[(size_t)lane * C + ch] : 0;
   int sh = (burn && lane < P) ? f.sumh[(size_t)lane * C + ch] : 0;
+  // lane j keeps the history word holding entry cnt % nadj of parameter j
+  int hw_idx = 0;
+  uint32_t hword = 0u;
+  if (burn && lane < P) {
+    hw_idx = (cnt % f.nadj) >> 5;
+    hword = f.hbits[((size_t)hw_idx * P + lane) * C + ch];
+  }
   const double beta = (double)f.beta[ibeta];
   float chi2c = f.chi2c[ch];
   int flag = 0, tlast = 0;
@@ -288,20 +297,24 @@ __device__ void step_one_chain(const FitDesc& f, int ch, int it, float2* ring, c
   unsigned nev = 0;
   const EhtView view = eht_view(f);
 
+  // This iteration's random numbers, substep s on lane s: the coordinate
+  // pick, the Gaussian step and log u of the accept test (drawn for every
+  // substep; counter-based, so unused draws shift nothing).
+  int k_l = 0;
+  double z_l = 0.0, lu_l = 0.0;
+  if (lane < P) {
+    const U4 r = philox4x32_10((uint32_t)ch, (uint32_t)it, (uint32_t)lane, 0u, f.key0, f.key1);
+    k_l = pick_with_retry(r.x, P, (uint32_t)ch, (uint32_t)it, (uint32_t)lane, 0u, f.key0, f.key1);
+    z_l = normal64(r.y, r.z);
+    lu_l = log((double)u01(r.w));
+  }
+
   for (int s = 0; s < P; ++s) {
-    const U4 r = philox4x32_10((uint32_t)ch, (uint32_t)it, (uint32_t)s, 0u, f.key0, f.key1);
-    const int k = pick_with_retry(r.x, P, (uint32_t)ch, (uint32_t)it, (uint32_t)s, 0u, f.key0, f.key1);
+    const int k = bcast(k_l, s);
     const int t = bcast(ivl, k);
-    // history entry that this proposal will overwrite (issued early)
-    int hold = 0;
-    size_t hidx = 0;
-    if (burn && lane == k) {
-      hidx = ((size_t)P * (size_t)(cnt % f.nadj) + (size_t)k) * C + ch;
-      hold = f.n_hist[hidx];
-    }
     const float old = bcast(pl, t);
     const float sd = bcast(stp, k);
-    double pd = (double)old + (double)sd * normal64(r.y, r.z);
+    double pd = (double)old + (double)sd * bcast(z_l, s);
     if (bcast(pdl, t) == 2) pd = pd - 6.283185307179586 * rint(pd / 6.283185307179586);
     const float pn = (float)pd;
     const bool inprior = !(pn < bcast(pminl, t) || pn > bcast(pmaxl, t));
@@ -313,8 +326,7 @@ __device__ void step_one_chain(const FitDesc& f, int ch, int it, float2* ring, c
       c2 = eht_chi2_wave<M, false>(view, q, ring, ch);
       ++nev;
       const double x = -0.5 * beta * (c2 - (double)chi2c);
-      const double lu = log((double)u01(r.w));
-      if (lu <= x) {
+      if (bcast(lu_l, s) <= x) {
         acc = 1;
         chi2c = (float)c2;
         if (lane == k) ++nac;
@@ -326,12 +338,21 @@ __device__ void step_one_chain(const FitDesc& f, int ch, int it, float2* ring, c
     told = old;
     tlast = t;
     if (burn && lane == k) {
-      f.n_hist[hidx] = acc;
+      // history entry cnt % nadj: the oldest one, replaced by this outcome
+      const uint32_t bit = 1u << ((cnt % f.nadj) & 31);
+      const int hold = (hword & bit) ? 1 : 0;
+      hword = acc ? (hword | bit) : (hword & ~bit);
       sh += acc - hold;
       const int den = cnt < f.nadj ? cnt : f.nadj;
       const float rate = (float)sh / (float)den;
       stp = rate < f.adapt_target ? (float)((double)stp / 1.01) : (float)((double)stp * 1.01);
       ++cnt;
+      const int w2 = (cnt % f.nadj) >> 5;
+      if (w2 != hw_idx) {
+        f.hbits[((size_t)hw_idx * P + lane) * C + ch] = hword;
+        hw_idx = w2;
+        hword = f.hbits[((size_t)hw_idx * P + lane) * C + ch];
+      }
     }
     if (tr.dec && it >= tr.it0 && it < tr.it0 + tr.nit && lane == 0) {
       const size_t ti = ((size_t)(it - tr.it0) * P + s) * C + ch;
@@ -345,7 +366,10 @@ __device__ void step_one_chain(const FitDesc& f, int ch, int it, float2* ring, c
     f.pstp[(size_t)lane * C + ch] = stp;
     f.n_cnt[(size_t)lane * C + ch] = cnt;
     f.n_acpt[(size_t)lane * C + ch] = nac;
-    if (burn) f.sumh[(size_t)lane * C + ch] = sh;
+    if (burn) {
+      f.sumh[(size_t)lane * C + ch] = sh;
+      f.hbits[((size_t)hw_idx * P + lane) * C + ch] = hword;
+    }
   }
   if (lane == 0) {
     f.chi2c[ch] = chi2c;
@@ -359,34 +383,43 @@ __device__ void step_one_chain(const FitDesc& f, int ch, int it, float2* ring, c
   }
 }
 
-// Next work item of the persistent step kernel: one atomic per wave (lane 0),
-// broadcast to the wave.  Every wave stops once the queue is exhausted.
-__device__ inline int grab_item(RunClock* clk, int qslot) {
+// Next work item of a persistent step kernel's queue q: one atomic per wave
+// (leader lane), broadcast to the wave.
+__device__ inline int grab_item(RunClock* clk, int q) {
   const int leader = __builtin_amdgcn_readfirstlane(lane_id());
   int v = 0;
-  if (lane_id() == leader) v = atomicAdd(&clk->qhead[qslot], 1);
+  if (lane_id() == leader) v = atomicAdd(&clk->qhead[q], 1);
   return __builtin_amdgcn_readfirstlane(v);
 }
 
+// Grid form: work-group b serves the range of XCD b % NXCD.  Persistent form:
+// each wave drains its own XCD's queue, then helps the other ranges in turn;
+// every queue head only grows, so every wave leaves once all are exhausted.
 template <class M, int WPB, bool PERSIST>
-__global__ void __launch_bounds__(64 * WPB) k_step_eht(const FitDesc* fits, const WorkItem* work, int nwork,
+__global__ void __launch_bounds__(64 * WPB) k_step_eht(const FitDesc* fits, const WorkItem* work, XcdRanges xr,
                                                        RunClock* clk, TraceBuf tr, int qslot) {
   __shared__ float2 ring_all[WPB][RING];
   const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   float2* ring = ring_all[wid];
   const int it = __builtin_amdgcn_readfirstlane(load_clock(clk));
+  const int xb = (int)(blockIdx.x % NXCD);
   if (!PERSIST) {
-    const int wv = blockIdx.x * WPB + wid;
-    if (wv >= nwork) return;
+    const int wv = xr.off[xb] + (int)(blockIdx.x / NXCD) * WPB + wid;
+    if (wv >= xr.off[xb + 1]) return;
     const WorkItem w = work[wv];
     const FitDesc& f = fits[w.fit];
     if (it >= f.nburn + f.niter) return;
     step_one_chain<M>(f, w.idx, it, ring, tr);
   } else {
-    for (int wv = grab_item(clk, qslot); wv < nwork; wv = grab_item(clk, qslot)) {
-      const WorkItem w = work[wv];
-      const FitDesc& f = fits[w.fit];
-      if (it < f.nburn + f.niter) step_one_chain<M>(f, w.idx, it, ring, tr);
+    for (int r = 0; r < NXCD; ++r) {
+      const int x = (xb + r) % NXCD;
+      const int base = xr.off[x], len = xr.off[x + 1] - base;
+      const int q = qslot * NXCD + x;
+      for (int j = grab_item(clk, q); j < len; j = grab_item(clk, q)) {
+        const WorkItem w = work[base + j];
+        const FitDesc& f = fits[w.fit];
+        if (it < f.nburn + f.niter) step_one_chain<M>(f, w.idx, it, ring, tr);
+      }
     }
   }
 }
@@ -492,9 +525,11 @@ __global__ void __launch_bounds__(256) k_step_quad(const FitDesc* fits, const Wo
     tlast = t;
     if (burn) {
       const int cnt = f.n_cnt[pk];
-      const size_t hidx = ((size_t)P * (size_t)(cnt % f.nadj) + (size_t)k) * C + ch;
-      const int hold = f.n_hist[hidx];
-      f.n_hist[hidx] = acc;
+      const int a = cnt % f.nadj;   // oldest history entry, replaced by this outcome
+      uint32_t* hw = f.hbits + ((size_t)(a >> 5) * P + k) * C + ch;
+      const uint32_t word = *hw, bit = 1u << (a & 31);
+      const int hold = (word & bit) ? 1 : 0;
+      *hw = acc ? (word | bit) : (word & ~bit);
       const int sh = f.sumh[pk] + acc - hold;
       f.sumh[pk] = sh;
       const int den = cnt < f.nadj ? cnt : f.nadj;
@@ -565,7 +600,7 @@ __device__ void exchange_wave(const FitDesc& f, int iseq, int it, bool burn, con
     f.tcur[me] = tc;
     if (!burn && nex) f.n_exch[me] += nex;
   }
-  if (!burn) {
+  if (!burn && f.record) {
     const int itr = it - f.nburn;
     if (slot) {
 #pragma unroll
@@ -604,7 +639,7 @@ __device__ void exchange_serial(const FitDesc& f, int iseq, int it, bool burn, c
     if (tr.ex && it >= tr.it0 && it < tr.it0 + tr.nit)
       tr.ex[((size_t)(it - tr.it0) * B + a) * S + iseq] = k | ((acc ? 1 : 0) << 16);
   }
-  if (!burn) {
+  if (!burn && f.record) {
     const int itr = it - f.nburn;
     for (int ib = 0; ib < B; ++ib) {
       const size_t me = (size_t)ib * S + iseq;
@@ -618,15 +653,19 @@ __device__ void exchange_serial(const FitDesc& f, int iseq, int it, bool burn, c
   }
 }
 
-__global__ void __launch_bounds__(64) k_exchange(const FitDesc* fits, const WorkItem* xwork, int nx,
+__global__ void __launch_bounds__(64) k_exchange(const FitDesc* fits, const WorkItem* xwork, XcdRanges xr,
                                                  RunClock* clk, TraceBuf tr) {
   const int it = load_clock(clk);
-  const WorkItem w = xwork[blockIdx.x];
-  const FitDesc& f = fits[w.fit];
-  if (it < f.nburn + f.niter) {
-    const bool burn = it < f.nburn;
-    if (f.nbeta <= 64) exchange_wave(f, w.idx, it, burn, tr);
-    else exchange_serial(f, w.idx, it, burn, tr);
+  const int xb = (int)(blockIdx.x % NXCD);
+  const int wv = xr.off[xb] + (int)(blockIdx.x / NXCD);
+  if (wv < xr.off[xb + 1]) {
+    const WorkItem w = xwork[wv];
+    const FitDesc& f = fits[w.fit];
+    if (it < f.nburn + f.niter) {
+      const bool burn = it < f.nburn;
+      if (f.nbeta <= 64) exchange_wave(f, w.idx, it, burn, tr);
+      else exchange_serial(f, w.idx, it, burn, tr);
+    }
   }
   // advance the iteration clock once every work-group of this launch is done
   if (threadIdx.x == 0) {
@@ -634,7 +673,7 @@ __global__ void __launch_bounds__(64) k_exchange(const FitDesc* fits, const Work
     const int v = atomicAdd(&clk->done, 1);
     if (v == (int)gridDim.x - 1) {
       atomicExch(&clk->done, 0);
-      for (int q = 0; q < NQSLOT; ++q) atomicExch(&clk->qhead[q], 0);
+      for (int q = 0; q < NQSLOT * NXCD; ++q) atomicExch(&clk->qhead[q], 0);
       __threadfence();
       atomicAdd(&clk->it, 1);
     }
@@ -665,21 +704,28 @@ hipError_t launch_eval_eht(int model, const FitDesc* fits, const WorkItem* work,
   return hipGetLastError();
 }
 
+// Work-groups of the grid form: NXCD x (longest range / WPB).
+static inline int xcd_grid(const XcdRanges& xr, int per_group) {
+  int mx = 0;
+  for (int x = 0; x < NXCD; ++x) mx = xr.off[x + 1] - xr.off[x] > mx ? xr.off[x + 1] - xr.off[x] : mx;
+  return NXCD * cdiv(mx, per_group);
+}
+
 template <class M>
 static void launch_step_m(const StepLaunch& L, hipStream_t s) {
   const TraceBuf tr{L.tr_dec, L.tr_chi2, L.tr_ex, L.tr_it0, L.tr_nit};
   if (L.persistent) {
     const int g = L.grid;
     switch (L.wpb) {
-      case 1: k_step_eht<M, 1, true><<<g, 64, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr, L.qslot); break;
-      case 2: k_step_eht<M, 2, true><<<g, 128, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr, L.qslot); break;
-      default: k_step_eht<M, 4, true><<<g, 256, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr, L.qslot); break;
+      case 1: k_step_eht<M, 1, true><<<g, 64, 0, s>>>(L.fits, L.work, L.xr, L.clk, tr, L.qslot); break;
+      case 2: k_step_eht<M, 2, true><<<g, 128, 0, s>>>(L.fits, L.work, L.xr, L.clk, tr, L.qslot); break;
+      default: k_step_eht<M, 4, true><<<g, 256, 0, s>>>(L.fits, L.work, L.xr, L.clk, tr, L.qslot); break;
     }
   } else {
     switch (L.wpb) {
-      case 1: k_step_eht<M, 1, false><<<L.nwork, 64, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr, L.qslot); break;
-      case 2: k_step_eht<M, 2, false><<<cdiv(L.nwork, 2), 128, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr, L.qslot); break;
-      default: k_step_eht<M, 4, false><<<cdiv(L.nwork, 4), 256, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr, L.qslot); break;
+      case 1: k_step_eht<M, 1, false><<<xcd_grid(L.xr, 1), 64, 0, s>>>(L.fits, L.work, L.xr, L.clk, tr, L.qslot); break;
+      case 2: k_step_eht<M, 2, false><<<xcd_grid(L.xr, 2), 128, 0, s>>>(L.fits, L.work, L.xr, L.clk, tr, L.qslot); break;
+      default: k_step_eht<M, 4, false><<<xcd_grid(L.xr, 4), 256, 0, s>>>(L.fits, L.work, L.xr, L.clk, tr, L.qslot); break;
     }
   }
 }
@@ -715,10 +761,10 @@ hipError_t launch_step_quad(int n, const StepLaunch& L, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_exchange(const StepLaunch& L, const WorkItem* xwork, int nx, hipStream_t s) {
-  if (nx <= 0) return hipSuccess;
+hipError_t launch_exchange(const StepLaunch& L, const WorkItem* xwork, const XcdRanges& xr, hipStream_t s) {
+  if (xr.off[NXCD] <= 0) return hipSuccess;
   const TraceBuf tr{L.tr_dec, L.tr_chi2, L.tr_ex, L.tr_it0, L.tr_nit};
-  k_exchange<<<nx, 64, 0, s>>>(L.fits, xwork, nx, L.clk, tr);
+  k_exchange<<<xcd_grid(xr, 1), 64, 0, s>>>(L.fits, xwork, xr, L.clk, tr);
   return hipGetLastError();
 }
 

@@ -4,10 +4,13 @@
 // All chain-state arrays use the reference's layouts (src/mcmc_interf.pyx:
 // 232-263) so they can be copied to and from the caller's buffers verbatim:
 //   pcur[nbeta][nseq][nptot], chi2c/flag/tcur/ptent/ptentn[nbeta][nseq],
-//   pstp/n_cnt/n_acpt[nprm][nbeta][nseq], n_hist[nadj][nprm][nbeta][nseq],
+//   pstp/n_cnt/n_acpt[nprm][nbeta][nseq],
 //   n_exch[nbeta][nseq], pout[nprm][nbeta][nseq][niter],
 //   tout[nbeta][nseq][niter], chi2[nseq][niter].
 // Chain index ist = nseq * ibeta + iseq (src/gpu_mcmc.cu:247).
+// The acceptance history n_hist[nadj][nprm][nbeta][nseq] (0/1 entries) is
+// kept as bits: hbits[hwords][nprm][nchain], entry a of (k, chain) is bit
+// a % 32 of word [a / 32][k][chain]; it is packed and unpacked on the host.
 #pragma once
 #include <stdint.h>
 
@@ -23,7 +26,8 @@ enum ModelKind : int {
 };
 
 constexpr int MAXP = 32;       // max nptot handled by the native kernels
-constexpr int RING = 256;      // per-wave LDS ring of model visibilities
+constexpr int RING = 512;      // per-wave LDS ring of model visibilities
+constexpr int MAX_CGROUP = 4;  // most tiles whose closures are formed together
 constexpr int TILE = 64;       // visibilities per wave step
 
 struct FitDesc {
@@ -36,19 +40,25 @@ struct FitDesc {
   int ncph;           // closure phases
   int ntile;          // ceil(nvis / TILE)
   int ring_ok;        // every closure lies within one ring window
+  int cgroup;         // closures are formed after every cgroup-th tile
   int nburn, niter;
   int nchain;         // nbeta * nseq
   uint32_t key0, key1;
   float adapt_target; // acceptance-rate threshold of the step adaptation (reference: 0.30)
   int quad_n, quad_neq;  // MODEL_QUADSYS sizes
+  int record;            // 1: write pout/tout/chi2 in the sampling stage
+  int hwords;            // words per acceptance history: ceil(nadj / 32)
   long long ndatm;       // length of datm (head nchain*ndat, then the phase tail)
 
   // ---- static inputs (device pointers)
-  const float4* vis;       // [nvis] (u, v, amp, 1/sigma^2) for the native kernel
-  const int4* clo;         // [ncph] vis indices (i, j, k, datum index) sorted by ring tile
-  const float4* clow;      // [ncph] (cos obs, sin obs, 1/sigma^2, 0)
-  const int2* tinfo;       // [ntile + 1] per tile: first closure handled after it,
-                           // and max u^2+v^2 of its visibilities (float bits)
+  const float4* vis;       // [(ntile + 1) * TILE] (u, v, amp, 1/sigma^2) for the native
+                           // kernel, zero past nvis
+  const int4* clo;         // [ncph + TILE] vis indices (i, j, k, datum index) sorted by
+                           // ring tile, zero past ncph
+  const float4* clow;      // [ncph + TILE] (cos obs, sin obs, 1/sigma^2, 0)
+  const int2* tinfo;       // [ntile + 2] per tile t: first closure whose largest index
+                           // lies past tile t - 1, and the max u^2+v^2 of tile t's
+                           // visibilities (float bits)
   const float* qcoef;      // MODEL_QUADSYS packed coefficients
   const float* coor; const float* dat; const float* std2r;
   const int* icoor; const int* idat;
@@ -57,7 +67,7 @@ struct FitDesc {
 
   // ---- chain state
   float* pcur; float* chi2c; float* pstp;
-  int* n_cnt; int* n_acpt; int* n_hist; int* n_exch; int* sumh;
+  int* n_cnt; int* n_acpt; uint32_t* hbits; int* n_exch; int* sumh;
   int* tcur; int* flag; float* ptent; int* ptentn;
 
   // ---- outputs
@@ -65,7 +75,6 @@ struct FitDesc {
 
   // ---- workspaces
   float* datm; float* chi2m;     // reference layout (generic path and eval outputs)
-  float2* vscratch;              // [nchain][nvis] model visibilities when !ring_ok
   CCalcModel* mc;                // device CCalcModel for the user's model()
 
   // ---- counters
@@ -73,13 +82,25 @@ struct FitDesc {
   int* errflag;                  // first negative model() return code
 };
 
+// Work lists of the wave-per-chain kernels are split into NXCD contiguous
+// ranges of nearly equal cost, one per XCD (each XCD has its own L2).  Work is
+// dealt to the ranges in chunks of XCHUNK consecutive chains (sequences) of
+// one fit, so that the 128-byte lines of the [..][chain] state arrays are
+// touched by one XCD only.  Work-group b runs on XCD b % NXCD (round-robin
+// dispatch) and takes its work from that XCD's range.
+constexpr int NXCD = 8;
+constexpr int XCHUNK = 32;
+struct XcdRanges {
+  int off[NXCD + 1];   // range x = [off[x], off[x + 1])
+};
+
 // Iteration clock shared by the step and exchange kernels so that a cycle of
 // launches can be captured once in a hipGraph and replayed.
 constexpr int NQSLOT = 6;
 struct RunClock {
-  int it;              // global iteration index (burn-in first, then sampling)
-  int done;            // exchange work-groups finished in the current iteration
-  int qhead[NQSLOT];   // work-queue heads of the persistent step kernels
+  int it;                     // global iteration index (burn-in first, then sampling)
+  int done;                   // exchange work-groups finished in the current iteration
+  int qhead[NQSLOT * NXCD];   // work-queue heads of the persistent step kernels, per XCD range
 };
 
 struct WorkItem {

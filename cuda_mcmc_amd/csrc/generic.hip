@@ -73,7 +73,7 @@ __global__ void __launch_bounds__(64) k_eval_user(const FitDesc* fits, const Wor
 }
 
 __global__ void __launch_bounds__(64) k_step_user(const FitDesc* fits, const WorkItem* work, int nwork,
-                                                  RunClock* clk) {
+                                                  RunClock* clk, TraceBuf tr) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (g >= nwork) return;
   const WorkItem w = work[g];
@@ -104,9 +104,10 @@ __global__ void __launch_bounds__(64) k_step_user(const FitDesc* fits, const Wor
     const float pn = (float)pd;
     const bool inprior = !(pn < f.pmint[t] || pn > f.pmaxt[t]);
     int acc = 0;
+    double c2 = 0.0;
     if (inprior) {
       *pt = pn;
-      const double c2 = user_chi2(f, ch, &err);
+      c2 = user_chi2(f, ch, &err);
       ++nev;
       const double x = -0.5 * beta * (c2 - (double)chi2c);
       if (log((double)u01(r.w)) <= x) {
@@ -120,11 +121,18 @@ __global__ void __launch_bounds__(64) k_step_user(const FitDesc* fits, const Wor
     flag = acc;
     told = old;
     tlast = t;
+    if (tr.dec && it >= tr.it0 && it < tr.it0 + tr.nit) {
+      const size_t ti = ((size_t)(it - tr.it0) * P + s) * C + ch;
+      tr.dec[ti] = k | ((inprior ? 1 : 0) << 8) | (acc << 9);
+      tr.chi2[ti] = inprior ? (float)c2 : 0.0f;
+    }
     if (burn) {
       const int cnt = f.n_cnt[pk];
-      const size_t hidx = ((size_t)P * (size_t)(cnt % f.nadj) + (size_t)k) * C + ch;
-      const int hold = f.n_hist[hidx];
-      f.n_hist[hidx] = acc;
+      const int a = cnt % f.nadj;   // oldest history entry, replaced by this outcome
+      uint32_t* hw = f.hbits + ((size_t)(a >> 5) * P + k) * C + ch;
+      const uint32_t word = *hw, bit = 1u << (a & 31);
+      const int hold = (word & bit) ? 1 : 0;
+      *hw = acc ? (word | bit) : (word & ~bit);
       const int sh = f.sumh[pk] + acc - hold;
       f.sumh[pk] = sh;
       const int den = cnt < f.nadj ? cnt : f.nadj;
@@ -156,7 +164,8 @@ hipError_t launch_eval_user(const FitDesc* fits, const WorkItem* work, int nwork
 
 hipError_t launch_step_user(const StepLaunch& L, hipStream_t s) {
   if (L.nwork <= 0) return hipSuccess;
-  k_step_user<<<cdiv(L.nwork, 64), 64, 0, s>>>(L.fits, L.work, L.nwork, L.clk);
+  const TraceBuf tr{L.tr_dec, L.tr_chi2, L.tr_ex, L.tr_it0, L.tr_nit};
+  k_step_user<<<cdiv(L.nwork, 64), 64, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr);
   return hipGetLastError();
 }
 

@@ -14,4 +14,9 @@ hipError_t launch_eht_grad9(const FitDesc* fd, const float* P, int n, int ldp, d
 hipError_t launch_eht_gn9(const FitDesc* fd, const float* P, int n, int ldp, double* chi2, double* jtr,
                           double* jtj, hipStream_t s);
 
+// Residuals r [rows][neq] and Jacobian J [rows][neq][n] of quadratic systems
+// x^T Q_e x + b_e . x - dat_e for parameter rows X [rows][n].
+hipError_t launch_quad_jac(const float* Q, const float* b, const float* dat, const int* sys, const float* X,
+                           int rows, int neq, int n, float* r, float* J, hipStream_t s);
+
 }  // namespace mhre

@@ -392,4 +392,40 @@ hipError_t launch_eht_gn9(const FitDesc* fd, const float* P, int n, int ldp, dou
   return hipGetLastError();
 }
 
+// ------------------------------------------------ quadratic systems (GN)
+
+// Residuals r_e = x^T Q_e x + b_e . x - dat_e (float64 accumulation, rounded)
+// and Jacobian J_ek = sum_j (Q_ekj + Q_ejk) x_j + b_ek of row i's system
+// sys[i] (sys == nullptr: system i).  One thread per row.
+__global__ void __launch_bounds__(256) k_quad_jac(const float* Q, const float* b, const float* dat, const int* sys,
+                                                  const float* X, int rows, int neq, int n, float* r, float* J) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rows) return;
+  const size_t s = sys ? (size_t)sys[i] : (size_t)i;
+  const float* x = X + (size_t)i * n;
+  for (int e = 0; e < neq; ++e) {
+    const float* q = Q + (s * neq + e) * (size_t)n * n;
+    const float* be = b + (s * neq + e) * (size_t)n;
+    double lhs = 0.0;
+    for (int j = 0; j < n; ++j) {
+      double row = 0.0;
+      for (int k = 0; k < n; ++k) row += (double)q[j * n + k] * (double)x[k];
+      lhs += ((double)be[j] + row) * (double)x[j];
+    }
+    r[(size_t)i * neq + e] = (float)(lhs - (double)dat[s * neq + e]);
+    for (int k = 0; k < n; ++k) {
+      float g = be[k];
+      for (int j = 0; j < n; ++j) g = fmaf(q[k * n + j] + q[j * n + k], x[j], g);
+      J[((size_t)i * neq + e) * n + k] = g;
+    }
+  }
+}
+
+hipError_t launch_quad_jac(const float* Q, const float* b, const float* dat, const int* sys, const float* X,
+                           int rows, int neq, int n, float* r, float* J, hipStream_t s) {
+  if (rows <= 0) return hipSuccess;
+  k_quad_jac<<<(rows + 255) / 256, 256, 0, s>>>(Q, b, dat, sys, X, rows, neq, n, r, J);
+  return hipGetLastError();
+}
+
 }  // namespace mhre

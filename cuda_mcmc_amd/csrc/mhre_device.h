@@ -5,7 +5,7 @@
 //    stream layout (counter = chain, iteration, substep, purpose) replaces the
 //    reference's per-thread cuRAND XORWOW states (src/gpu_mcmc.cu:228-239).
 //  * fp32 special functions used by the visibility model, written from their
-//    mathematical definitions: Cody-Waite reduced sincos with Taylor kernels,
+//    mathematical definitions: Cody-Waite reduced sincos with least-squares kernels,
 //    atan2 from a Chebyshev-fitted odd polynomial, and the Bessel-derived
 //    B(y) = J1(x)/x and C(y) = J2(x)/x^2 (y = x^2) from Chebyshev-fitted
 //    polynomials on x <= 8 and Hankel's asymptotic expansion beyond.
@@ -83,7 +83,8 @@ __device__ inline float horner(const float (&c)[N], float t) {
 }
 
 // sin and cos of theta: k = rint(theta/pi), r = theta - k*pi in [-pi/2, pi/2]
-// (two-constant Cody-Waite), Taylor kernels to r^13 / r^14, sign (-1)^k.
+// (two-constant Cody-Waite), least-squares odd / even kernels of degree 9 / 10,
+// sign (-1)^k.
 __device__ inline void sincos_fast(float theta, float* s, float* c) {
   const float PI_HI = 3.14159274101257324219f;      // fp32(pi)
   const float PI_LO = -8.74227800037248274685e-08f; // pi - PI_HI
@@ -151,6 +152,12 @@ __device__ inline float bcast(float v, int lane) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
 }
 __device__ inline int bcast(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
+__device__ inline double bcast(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 
 __device__ inline double wave_sum(double v) {
 #pragma unroll

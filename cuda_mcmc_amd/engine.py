@@ -49,7 +49,7 @@ class Fit:
     def __init__(self, pdescr, pmint, pmaxt, coor, dat, std2r, nbeta=32, nseq=14, nburn=300, niter=500,
                  model="auto", imodel=0, icoor=None, idat=None, ptotal=None, beta=None, beta1=1.0,
                  betan=1e-4, nadj=100, npass=1, seed=0, adapt_target=0.30, pstp=None,
-                 init_chains=True, model_data=False, ndatm=None, it0=0):
+                 init_chains=True, model_data=False, ndatm=None, it0=0, record=True):
         f32, i32 = np.float32, np.int32
         self.model = MODELS[model] if isinstance(model, str) else int(model)
         self.imodel = int(imodel)
@@ -91,9 +91,12 @@ class Fit:
         self.chi2c = np.zeros((B, S), f32)
         self.ptent = np.zeros((B, S), f32)
         self.ptentn = np.zeros((B, S), i32)
-        self.pout = np.zeros((P, B, S, self.niter), f32)
-        self.tout = np.zeros((B, S, self.niter), i32)
-        self.chi2 = np.zeros((S, self.niter), f32)
+        if record:
+            self.pout = np.zeros((P, B, S, self.niter), f32)
+            self.tout = np.zeros((B, S, self.niter), i32)
+            self.chi2 = np.zeros((S, self.niter), f32)
+        else:   # sampling-stage states not recorded; only the final state comes back
+            self.pout = self.tout = self.chi2 = None
         self.rndst = np.zeros((B, S, 48), np.uint8)
         if model_data:
             nvis = int(self.idat[0]) if self.idat.size else 0
@@ -173,6 +176,14 @@ class Session:
         _lib.check(self.L, self.L.mhre_session_time_kernels(self.h, int(niter), C.byref(a), C.byref(b)),
                    "mhre_session_time_kernels")
         return a.value, b.value
+
+    def autotune(self, iters=8):
+        """Pick the step kernels' launch form by timing 6 candidates for `iters`
+        real iterations each; returns (waves_per_block, persistent, step_ms)."""
+        w, p, t = C.c_int(), C.c_int(), C.c_double()
+        _lib.check(self.L, self.L.mhre_session_autotune(self.h, int(iters), C.byref(w), C.byref(p), C.byref(t)),
+                   "mhre_session_autotune")
+        return w.value, p.value, t.value
 
     def sync(self):
         _lib.check(self.L, self.L.mhre_session_sync(self.h), "mhre_session_sync")

@@ -102,7 +102,8 @@ typedef struct MhreFitSpec {
   float *datm, *chi2m;          /* may be NULL: final-state model data not wanted */
   int *pdescr, *ivar, *invar;
   float *ptotal, *pmint, *pmaxt;
-  float *pout; int *tout; float *chi2;
+  float *pout; int *tout; float *chi2;  /* all NULL: sampling-stage states are
+                                           not recorded (final state only) */
   float *chi2c; int *tcur; int *flag;
   int *n_acpt, *n_exch, *n_cnt, *n_hist;
   float *beta, *pstp, *ptent; int *ptentn;
@@ -149,6 +150,13 @@ int mhre_session_run(void *session, int niter);
  * stream; returns the average device time per iteration of the step
  * kernel(s) and of the exchange kernel, in milliseconds. */
 int mhre_session_time_kernels(void *session, int niter, double *step_ms, double *exch_ms);
+/* Choose the step kernels' launch form (waves per work-group 1/2/4, grid or
+ * persistent work queue) by timing each candidate on `iters` real
+ * iterations; the 6 * iters iterations advance the chains like any others
+ * (every form gives bit-identical chains).  Returns the choice and its step
+ * time per iteration.  No reference counterpart. */
+int mhre_session_autotune(void *session, int iters, int *waves_per_block, int *persistent,
+                          double *step_ms);
 /* Wait for the session's stream. */
 int mhre_session_sync(void *session);
 /* Copy chain state and outputs back into the fit specs' host arrays; with
@@ -195,6 +203,15 @@ int mhre_eht_chi2_grad(void *handle, const float *params, int n, int ldp,
 int mhre_eht_gn_system(void *handle, const float *params, int n, int ldp,
                        double *chi2, double *jtr, double *jtj, void *stream);
 int mhre_eht_data_destroy(void *handle);
+
+/* Residuals r[rows][neq] = x^T Q_e x + b_e . x - dat_e and Jacobian
+ * J[rows][neq][n] of the quadratic systems Q[nsys][neq][n][n],
+ * b[nsys][neq][n], dat[nsys][neq] at the rows X[rows][n]; row i belongs to
+ * system sys[i] (sys == NULL: system i).  Host pointers, synchronous.  Drives
+ * the Gauss-Newton polish of solved systems; no reference counterpart (the
+ * reference's solvsys example is sampled only, examples/solvsys/model.cuh). */
+int mhre_quad_jacobian(const float *Q, const float *b, const float *dat, const int *sys,
+                       int nsys, int neq, int n, const float *X, int rows, float *r, float *J);
 
 #ifdef __cplusplus
 }

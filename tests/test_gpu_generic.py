@@ -1,0 +1,80 @@
+"""The reference's model.cuh contract and the native quadratic-system kernel,
+in lock step with the float64 oracle (same Philox streams):
+
+* Rosenbrock (examples/search_Rosenbrock/model.cuh) and solvsys
+  (examples/solvsys/model.cuh) through the generic model() path of their own
+  per-project libraries;
+* systems of 8 quadratic equations in 8 unknowns on the native kernel;
+* Gauss-Newton polish of quadratic systems from near a planted root, and a
+  small end-to-end solve (MH-RE, best chain, polish).
+Decisions identical except inside the tie band; counters, histories and step
+sizes bit-identical on the sequences without a tie.  The quadratic-system
+kernel evaluates x^T Q x in fp32 (about 1e-6 of the largest term), so its
+tie band is 3e-5 of chi^2 instead of 2e-6.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from cuda_mcmc_amd import quadsys, synthdata
+from cuda_mcmc_amd.engine import Fit
+from oracle.chi2 import generic_chi2
+from tests.test_gpu_mhre import _lockstep
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _lib(name):
+    p = os.path.join(ROOT, "examples", name, "libmhre.so")
+    if not os.path.isfile(p):
+        pytest.fail("example library %s not built" % p)
+    return p
+
+
+def test_lockstep_rosenbrock_generic():
+    coor = np.zeros(1, np.float32)
+    fit = Fit([1, 1], [-5, -5], [5, 5], coor, [0.0], [1.0], nbeta=32, nseq=16, nburn=4, niter=4,
+              model="user", npass=1, seed=9, pstp=0.5, adapt_target=0.25,
+              beta=np.geomspace(1.0, 1e-3, 32).astype(np.float32))
+    f = lambda P: generic_chi2("rosenbrock", P, coor, fit.dat, fit.std2r)
+    compared, _ = _lockstep(fit, f, 8, lib=_lib("search_Rosenbrock"))
+    assert compared >= 8 * 16 * 0.5
+
+
+def test_lockstep_solvsys_generic():
+    coor = np.array([2, 2, 0.25, 0.5], np.float32)
+    fit = Fit([1, 1], [-5, -5], [5, 5], coor, [1.0, 0.5], [1.0, 1.0], nbeta=32, nseq=14, nburn=4, niter=4,
+              model="user", npass=1, seed=4)
+    f = lambda P: generic_chi2("solvsys", P, coor, fit.dat, fit.std2r)
+    compared, _ = _lockstep(fit, f, 8, lib=_lib("solvsys"))
+    assert compared >= 8 * 14 * 0.5
+
+
+def test_lockstep_quadsys_native():
+    ds = synthdata.quadsys_dataset(nsys=1, seed=3)
+    fit = quadsys.quad_fit(ds["Q"][0], ds["b"][0], ds["c"][0], nbeta=16, nseq=32, nburn=4, niter=3, seed=21,
+                           sigma=1.0)
+    f = lambda P: generic_chi2("quadsys", P, fit.coor, fit.dat, fit.std2r)
+    compared, _ = _lockstep(fit, f, 7, tie_rel=3e-5)
+    assert compared >= 7 * 32 * 0.5
+
+
+def test_quad_gauss_newton_from_near_root():
+    ds = synthdata.quadsys_dataset(nsys=32, seed=8)
+    rng = np.random.default_rng(0)
+    X0 = ds["xstar"] + 0.05 * rng.standard_normal(ds["xstar"].shape)
+    X, resid = quadsys.gauss_newton(ds["Q"], ds["b"], -ds["c"], X0, iters=25)
+    print("residual norms", np.sort(resid)[-5:])
+    assert np.all(resid < 1e-5)
+    np.testing.assert_allclose(X, ds["xstar"], atol=1e-4)
+
+
+def test_quad_solve_end_to_end():
+    ds = synthdata.quadsys_dataset(nsys=32, seed=9)
+    out = quadsys.solve_systems(ds, nbeta=16, nseq=32, nburn=1500, gn_iters=25, seed=100)
+    ok = out["resid"] < 1e-5
+    print("solved %d of %d; best chi2 median %.3g" % (ok.sum(), ok.size, np.median(out["chi2_best"])))
+    assert ok.mean() >= 0.75
