@@ -18,7 +18,8 @@ Also reported:
   cpu_baseline  the float64 C/OpenMP oracle port on the host cores;
   batched       64 uneven fits (config 3 shapes) in one persistent session,
                 launch form picked by the autotuner, vs the same fits one
-                session at a time;
+                session at a time (fused native kernel, and the per-datum
+                model.cuh contract path);
   adjoint       chi^2 + gradient of 4096 parameter points against 20000
                 visibilities + 6000 closure phases (fp32, HIP events), and a
                 Gauss-Newton polish + Laplace covariance of short fits;
@@ -142,7 +143,23 @@ def batched_vs_plain(iters, nfits, log_every=16):
             tp += time.time() - t0
             plain_vis += s.stats()["vis_evals"] - e0["vis_evals"]
     plain = plain_vis / tp
+    # the same fits one at a time through the reference's per-datum model()
+    # contract (generic model.cuh path, one lane per chain)
+    tg = 0.0
+    gen_vis = 0.0
+    for f, (d, nb, ns) in enumerate(fits):
+        with Session([mk(d, nb, ns, f)], force_generic=True) as s:
+            s.run(1)
+            s.sync()
+            e0 = s.stats()
+            t0 = time.time()
+            s.run(iters)
+            s.sync()
+            tg += time.time() - t0
+            gen_vis += s.stats()["vis_evals"] - e0["vis_evals"]
+    generic = gen_vis / tg
     return dict(fits=nfits, iterations=iters, autotuned={"waves_per_block": wpb, "persistent": pers},
+                plain_generic_sequential_gvis_per_s=generic / 1e9, ratio_vs_generic=batched / generic,
                 chains=int(sum(nb * ns for _, nb, ns in fits)),
                 vis_total=int(sum(d["nvis"] for d, _, _ in fits)),
                 batched_gvis_per_s=batched / 1e9, plain_sequential_gvis_per_s=plain / 1e9,
@@ -253,6 +270,15 @@ def main():
               "no multi-GPU figure is produced.", file=sys.stderr)
         return 2
 
+    if not args.no_adjoint:
+        # torch's HIP runtime is brought up before the engine library's, as a
+        # user of the autograd interface would do
+        import torch
+        if not torch.cuda.is_available():
+            log("torch sees no GPU: the adjoint leg is skipped")
+            args.no_adjoint = True
+        else:
+            torch.zeros(1, device="cuda")
     from cuda_mcmc_amd import synthdata, mcmc_interf
     from cuda_mcmc_amd.engine import Session
 

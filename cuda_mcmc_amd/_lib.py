@@ -102,7 +102,7 @@ def load(path=None):
 def _declare(L):
     L.mcmc_cuda.restype = C.c_int
     L.mcmc_cuda.argtypes = ([fp, fp, fp, ip, ip, fp, fp, ip, ip, ip, fp, fp, fp, fp, ip, fp, fp, ip, ip,
-                             ip, ip, ip, ip, fp, fp, fp, ip, fp, up, C.c_ulonglong] + [C.c_int] * 15)
+                             ip, ip, ip, ip, fp, fp, fp, ip, fp, up, C.c_ulonglong] + [C.c_int] * 14)
     L.calc_modchi2.restype = C.c_int
     L.calc_modchi2.argtypes = [fp, fp, fp, ip, ip, fp, fp, fp, fp, ip] + [C.c_int] * 9
     L.calc_model.restype = C.c_int

@@ -477,41 +477,52 @@ int launch_iteration(Session* S) {
 
 // Iterations with HIP events around every launch (on the session stream):
 // average device time per launch of the step kernels and of the exchange.
+// All launches and events of a batch are queued before the host waits, so
+// the gaps between launches are not counted.
 int time_kernels(Session* S, int niter, double* step_ms, double* exch_ms) {
   (void)hipGetLastError();
-  hipEvent_t e[3];
-  for (int i = 0; i < 3; ++i) HIPCHK(hipEventCreate(&e[i]));
+  constexpr int BATCH = 64;
+  std::vector<hipEvent_t> ev(3 * BATCH, nullptr);
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  struct Destroy {
+    std::vector<hipEvent_t>& v;
+    ~Destroy() { for (auto e : v) if (e) hipEventDestroy(e); }
+  } destroy{ev};
   double st = 0.0, ex = 0.0;
-  int rc = MHRE_OK;
-  for (int it = 0; it < niter && rc == MHRE_OK; ++it) {
-    HIPCHK(hipEventRecord(e[0], S->stream));
-    for (auto& k : S->kinds) {
-      const StepLaunch L = make_launch(S, k);
-      hipError_t err = hipSuccess;
-      if (k.kind == K_SGRA9) err = launch_step_eht(MODEL_SGRA9, L, S->stream);
-      else if (k.kind == K_SGRA13) err = launch_step_eht(MODEL_SGRA13, L, S->stream);
-      else if (k.kind == K_QUAD) err = launch_step_quad(k.quad_n, L, S->stream);
-      else err = launch_step_user(L, S->stream);
-      HIPCHK(err);
+  for (int done = 0; done < niter;) {
+    const int nb = std::min(BATCH, niter - done);
+    for (int i = 0; i < nb; ++i) {
+      HIPCHK(hipEventRecord(ev[3 * i], S->stream));
+      for (auto& k : S->kinds) {
+        const StepLaunch L = make_launch(S, k);
+        hipError_t err = hipSuccess;
+        if (k.kind == K_SGRA9) err = launch_step_eht(MODEL_SGRA9, L, S->stream);
+        else if (k.kind == K_SGRA13) err = launch_step_eht(MODEL_SGRA13, L, S->stream);
+        else if (k.kind == K_QUAD) err = launch_step_quad(k.quad_n, L, S->stream);
+        else err = launch_step_user(L, S->stream);
+        HIPCHK(err);
+      }
+      HIPCHK(hipEventRecord(ev[3 * i + 1], S->stream));
+      StepLaunch L{};
+      L.fits = S->d_fits;
+      L.clk = S->d_clk;
+      HIPCHK(launch_exchange(L, S->d_x, S->xr_x, S->stream));
+      HIPCHK(hipEventRecord(ev[3 * i + 2], S->stream));
     }
-    HIPCHK(hipEventRecord(e[1], S->stream));
-    StepLaunch L{};
-    L.fits = S->d_fits;
-    L.clk = S->d_clk;
-    HIPCHK(launch_exchange(L, S->d_x, S->xr_x, S->stream));
-    HIPCHK(hipEventRecord(e[2], S->stream));
-    HIPCHK(hipEventSynchronize(e[2]));
-    float a = 0.0f, b = 0.0f;
-    HIPCHK(hipEventElapsedTime(&a, e[0], e[1]));
-    HIPCHK(hipEventElapsedTime(&b, e[1], e[2]));
-    st += a;
-    ex += b;
+    HIPCHK(hipEventSynchronize(ev[3 * (nb - 1) + 2]));
+    for (int i = 0; i < nb; ++i) {
+      float a = 0.0f, b = 0.0f;
+      HIPCHK(hipEventElapsedTime(&a, ev[3 * i], ev[3 * i + 1]));
+      HIPCHK(hipEventElapsedTime(&b, ev[3 * i + 1], ev[3 * i + 2]));
+      st += a;
+      ex += b;
+    }
+    done += nb;
+    S->iters_done += nb;
   }
-  for (int i = 0; i < 3; ++i) hipEventDestroy(e[i]);
-  S->iters_done += niter;
   if (step_ms) *step_ms = niter > 0 ? st / niter : 0.0;
   if (exch_ms) *exch_ms = niter > 0 ? ex / niter : 0.0;
-  return rc;
+  return MHRE_OK;
 }
 
 void drop_graph(Session* S) {
@@ -975,6 +986,12 @@ int stats(Session* S, MhreStats* st) {
     if (h.kind == K_SGRA9 || h.kind == K_SGRA13) {
       st->vis_evals += (double)ev[2 * i] * h.nvis_k;
       st->cph_evals += (double)ev[2 * i] * h.ncph;
+    } else if (h.kind == K_USER && (h.s.imodel == 1 || h.s.imodel == 2) && h.s.idat && h.s.nidat >= 1) {
+      // the EHT model.cuh on the generic path: idat = [nvis, ndatm, cpExt]
+      const int nv = h.s.idat[0], nc = h.s.ndat - nv;
+      const int cpext = h.s.nidat >= 3 ? h.s.idat[2] : 0;
+      st->vis_evals += (double)ev[2 * i] * (nv + (cpext ? 3 * nc : 0));
+      st->cph_evals += (double)ev[2 * i] * nc;
     }
   }
   st->iters_done = S->iters_done;

@@ -46,18 +46,21 @@ __device__ inline void ring_coefs(double Rex, double rq, double fade, double d, 
 
 // Ring visibility Fr = Fex - Fin at rotated coordinates (U, V).
 // any_large == false: the caller guarantees ye <= 64 on every lane, so only
-// the polynomial branch of B, C is evaluated.
-__device__ inline float2 ring_vis(const RingInv& q, float U, float V, bool any_large) {
+// the y <= 64 evaluation bc of B, C runs (BesselPoly or BesselTab).
+template <class BC>
+__device__ inline float2 ring_vis(const RingInv& q, float U, float V, bool any_large, const BC& bc) {
   const float rho2 = fmaf(U, U, V * V);
   const float ye = q.se * rho2;
   const float yi = q.rq2 * ye;
   float Be, Ce, Bi, Ci;
   if (!any_large) {
-    bessel_BC_poly(ye, &Be, &Ce);
-    bessel_BC_poly(yi, &Bi, &Ci);
+    bc(ye, &Be, &Ce);
+    bc(yi, &Bi, &Ci);
   } else {
-    bessel_BC(ye, &Be, &Ce);
-    bessel_BC(yi, &Bi, &Ci);
+    if (ye <= 64.0f) bc(ye, &Be, &Ce);
+    else bessel_BC_asym(ye, &Be, &Ce);
+    if (yi <= 64.0f) bc(yi, &Bi, &Ci);
+    else bessel_BC_asym(yi, &Bi, &Ci);
   }
   const float ree = q.kBe * Be;
   const float ime = q.kCe * U * Ce;
@@ -106,10 +109,11 @@ struct Sgra9 {
   }
 
 
-  __device__ static float2 vis(const Inv& q, float u, float v, bool any_large) {
+  template <class BC = BesselPoly>
+  __device__ static float2 vis(const Inv& q, float u, float v, bool any_large, const BC& bc = BC()) {
     const float U = fmaf(u, q.cth, v * q.sth);
     const float V = fmaf(v, q.cth, -u * q.sth);
-    const float2 F = ring_vis(q.r, U, V, any_large);
+    const float2 F = ring_vis(q.r, U, V, any_large, bc);
     float2 out;
     if (q.r.gauss) {
       const float ga = __builtin_amdgcn_exp2f(-fmaf(q.gaU, U * U, q.gaV * (V * V)));
@@ -167,12 +171,13 @@ struct Sgra13 {
   }
 
 
-  __device__ static float2 vis(const Inv& q, float u, float v, bool any_large) {
+  template <class BC = BesselPoly>
+  __device__ static float2 vis(const Inv& q, float u, float v, bool any_large, const BC& bc = BC()) {
     float2 out = make_float2(0.0f, 0.0f);
     if (!q.gonly) {
       const float U = fmaf(u, q.cth, v * q.sth);
       const float V = fmaf(v, q.cth, -u * q.sth);
-      const float2 F = ring_vis(q.r, U, V, any_large);
+      const float2 F = ring_vis(q.r, U, V, any_large, bc);
       out.x = q.r.cr * F.x;
       out.y = q.r.cr * F.y;
     }

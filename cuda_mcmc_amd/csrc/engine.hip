@@ -140,7 +140,8 @@ __device__ inline float closure_term(float2 a, float2 b, float2 e, float4 ow, fl
 // The fp32 terms are accumulated in fp64 per lane and summed across the wave.
 // OUT: also write datm/chi2m of chain ch in the reference layout.
 template <class M, bool OUT>
-__device__ double eht_chi2_wave(const EhtView& v, const typename M::Inv& q, float2* ring, int ch) {
+__device__ double eht_chi2_wave(const EhtView& v, const typename M::Inv& q, float2* ring, const BesselTab& bt,
+                                int ch) {
   const int lane = lane_id();
   const bool clos = v.ncph > 0 && v.ring_ok;
   const bool out = OUT && v.datm != nullptr;
@@ -163,7 +164,7 @@ __device__ double eht_chi2_wave(const EhtView& v, const typename M::Inv& q, floa
     int4 ix = ldg(v.clo, cf);
     float4 ow = ldg(v.clow, cf);
     const bool any_large = q.r.se * r2cur > 64.0f;
-    const float2 V = M::vis(q, d.x, d.y, any_large);
+    const float2 V = M::vis(q, d.x, d.y, any_large, bt);
     const float amp = __builtin_amdgcn_sqrtf(fmaf(V.x, V.x, V.y * V.y));
     const float r = d.z - amp;
     acc += (double)(r * r * d.w);   // zero weight past the amplitude data
@@ -213,8 +214,8 @@ __device__ double eht_chi2_wave(const EhtView& v, const typename M::Inv& q, floa
       const float4 ow = ldg(v.clow, c);
       const float4 da = ldg(v.vis, ix.x), db = ldg(v.vis, ix.y), de = ldg(v.vis, ix.z);
       float mc;
-      const float term = closure_term(M::vis(q, da.x, da.y, true), M::vis(q, db.x, db.y, true),
-                                      M::vis(q, de.x, de.y, true), ow, out ? &mc : nullptr);
+      const float term = closure_term(M::vis(q, da.x, da.y, true, bt), M::vis(q, db.x, db.y, true, bt),
+                                      M::vis(q, de.x, de.y, true, bt), ow, out ? &mc : nullptr);
       acc += (double)term;
       if (out) {
         const size_t imd = (size_t)ch * v.ndat + ix.w;
@@ -242,6 +243,9 @@ template <class M, bool OUT>
 __global__ void __launch_bounds__(256) k_eval_eht(const FitDesc* fits, const WorkItem* work, int nwork,
                                                   int respect_flag) {
   __shared__ float2 ring_all[4][RING];
+  __shared__ float4 btab[2 * BESSEL_TAB_N];
+  load_bessel_tab(btab);
+  const BesselTab bt{btab};
   const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int wv = blockIdx.x * 4 + wid;
   if (wv >= nwork) return;
@@ -260,14 +264,15 @@ __global__ void __launch_bounds__(256) k_eval_eht(const FitDesc* fits, const Wor
   }
   const float pl = lane < f.nptot ? f.pcur[(size_t)ch * f.nptot + lane] : 0.0f;
   const typename M::Inv q = prepare_from_lanes<M>(pl);
-  const double c2 = eht_chi2_wave<M, OUT>(eht_view(f), q, ring, ch);
+  const double c2 = eht_chi2_wave<M, OUT>(eht_view(f), q, ring, bt, ch);
   if (lane == 0) f.chi2c[ch] = (float)c2;
 }
 
 // ------------------------------------------------------------ step kernel
 
 template <class M>
-__device__ void step_one_chain(const FitDesc& f, int ch, int it, float2* ring, const TraceBuf& tr) {
+__device__ void step_one_chain(const FitDesc& f, int ch, int it, float2* ring, const BesselTab& bt,
+                               const TraceBuf& tr) {
   const int lane = lane_id();
   const int C = f.nchain, P = f.nprm, N = f.nptot;
   const int ibeta = ch / f.nseq;
@@ -323,7 +328,7 @@ __device__ void step_one_chain(const FitDesc& f, int ch, int it, float2* ring, c
     if (inprior) {
       if (lane == t) pl = pn;
       const typename M::Inv q = prepare_from_lanes<M>(pl);
-      c2 = eht_chi2_wave<M, false>(view, q, ring, ch);
+      c2 = eht_chi2_wave<M, false>(view, q, ring, bt, ch);
       ++nev;
       const double x = -0.5 * beta * (c2 - (double)chi2c);
       if (bcast(lu_l, s) <= x) {
@@ -399,6 +404,9 @@ template <class M, int WPB, bool PERSIST>
 __global__ void __launch_bounds__(64 * WPB) k_step_eht(const FitDesc* fits, const WorkItem* work, XcdRanges xr,
                                                        RunClock* clk, TraceBuf tr, int qslot) {
   __shared__ float2 ring_all[WPB][RING];
+  __shared__ float4 btab[2 * BESSEL_TAB_N];
+  load_bessel_tab(btab);
+  const BesselTab bt{btab};
   const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   float2* ring = ring_all[wid];
   const int it = __builtin_amdgcn_readfirstlane(load_clock(clk));
@@ -409,7 +417,7 @@ __global__ void __launch_bounds__(64 * WPB) k_step_eht(const FitDesc* fits, cons
     const WorkItem w = work[wv];
     const FitDesc& f = fits[w.fit];
     if (it >= f.nburn + f.niter) return;
-    step_one_chain<M>(f, w.idx, it, ring, tr);
+    step_one_chain<M>(f, w.idx, it, ring, bt, tr);
   } else {
     for (int r = 0; r < NXCD; ++r) {
       const int x = (xb + r) % NXCD;
@@ -418,7 +426,7 @@ __global__ void __launch_bounds__(64 * WPB) k_step_eht(const FitDesc* fits, cons
       for (int j = grab_item(clk, q); j < len; j = grab_item(clk, q)) {
         const WorkItem w = work[base + j];
         const FitDesc& f = fits[w.fit];
-        if (it < f.nburn + f.niter) step_one_chain<M>(f, w.idx, it, ring, tr);
+        if (it < f.nburn + f.niter) step_one_chain<M>(f, w.idx, it, ring, bt, tr);
       }
     }
   }
@@ -568,13 +576,21 @@ __device__ void exchange_wave(const FitDesc& f, int iseq, int it, bool burn, con
   int perm = lane;
   int nex = 0;
   if (B > 1) {
+    // attempt a's pair pick and log u on lane a, all drawn up front
+    int k_l = 0;
+    double lu_l = 0.0, hb_l = 0.0;
+    if (slot) {
+      const U4 r = philox4x32_10((uint32_t)iseq, (uint32_t)it, (uint32_t)lane, RNG_EXCH, f.key0, f.key1);
+      k_l = pick_with_retry(r.x, B - 1, (uint32_t)iseq, (uint32_t)it, (uint32_t)lane, RNG_EXCH, f.key0, f.key1);
+      lu_l = log((double)u01(r.y));
+    }
+    // -0.5 (beta[k+1] - beta[k]) on lane k
+    if (lane < B - 1) hb_l = -0.5 * ((double)f.beta[lane + 1] - (double)f.beta[lane]);
     for (int a = 0; a < B; ++a) {
-      const U4 r = philox4x32_10((uint32_t)iseq, (uint32_t)it, (uint32_t)a, RNG_EXCH, f.key0, f.key1);
-      const int k = pick_with_retry(r.x, B - 1, (uint32_t)iseq, (uint32_t)it, (uint32_t)a, RNG_EXCH,
-                                    f.key0, f.key1);
+      const int k = bcast(k_l, a);
       const float ck = bcast(c2, k), ck1 = bcast(c2, k + 1);
-      const double x = -0.5 * ((double)f.beta[k + 1] - (double)f.beta[k]) * ((double)ck - (double)ck1);
-      const bool acc = log((double)u01(r.y)) <= x;
+      const double x = bcast(hb_l, k) * ((double)ck - (double)ck1);
+      const bool acc = bcast(lu_l, a) <= x;
       if (acc) {
         const int tk = bcast(tc, k), tk1 = bcast(tc, k + 1);
         const int pk = bcast(perm, k), pk1 = bcast(perm, k + 1);
@@ -667,14 +683,15 @@ __global__ void __launch_bounds__(64) k_exchange(const FitDesc* fits, const Work
       else exchange_serial(f, w.idx, it, burn, tr);
     }
   }
-  // advance the iteration clock once every work-group of this launch is done
+  // Advance the iteration clock once every work-group of this launch has
+  // counted itself.  Only the next launches read the clock and the queue
+  // heads, and the kernel boundary orders them after these atomics, so no
+  // fence is needed here.
   if (threadIdx.x == 0) {
-    __threadfence();
     const int v = atomicAdd(&clk->done, 1);
     if (v == (int)gridDim.x - 1) {
       atomicExch(&clk->done, 0);
       for (int q = 0; q < NQSLOT * NXCD; ++q) atomicExch(&clk->qhead[q], 0);
-      __threadfence();
       atomicAdd(&clk->it, 1);
     }
   }

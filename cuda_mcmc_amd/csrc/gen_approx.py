@@ -9,6 +9,10 @@ BESSEL_B  B(y) = J1(x)/x, y = x^2 in [0, 64]: Chebyshev interpolation in
 BESSEL_C  C(y) = J2(x)/x^2 on the same range and variable.
 BESSEL_D  D(y) = J3(x)/x^3 on the same range and variable (derivative
           dC/dy = -D/2, used by the chi^2 gradient kernels).
+BESSEL_TAB  piecewise cubics of B and C on the unit intervals [j, j+1) of y,
+          j = 0..64, in s = y - j (least squares at Chebyshev nodes): 8
+          floats per interval (B c0..c3, C c0..c3), read from LDS by the
+          fused step kernel.
 HANKEL    coefficients of Hankel's asymptotic expansion (Abramowitz & Stegun
           9.2.5, 9.2.9, 9.2.10) for J1 and J2, used for x > 8.
 SIN/COS   least-squares odd / even polynomials for sin and cos on
@@ -126,6 +130,26 @@ def atan_fit(deg_odd_terms):
     return q.coef
 
 
+def piecewise_cubic(f, nint, h=1.0, deg=3):
+    """Per-interval least-squares cubics: [nint][deg+1] in s = (y - j h)/h."""
+    s = np.cos(np.pi * (np.arange(64) + 0.5) / 64) * 0.5 + 0.5
+    out = []
+    for j in range(nint):
+        out.append(np.polynomial.polynomial.polyfit(s, f((j + s) * h), deg))
+    return np.array(out)
+
+
+def check_piecewise32(c, f, h=1.0, upto=64.0):
+    worst = 0.0
+    ss = np.linspace(0.0, 1.0, 2001).astype(np.float32)
+    for j in range(int(upto / h)):
+        acc = np.full_like(ss, np.float32(c[j][-1]))
+        for k in range(c.shape[1] - 2, -1, -1):
+            acc = (acc * ss + np.float32(c[j][k])).astype(np.float32)
+        worst = max(worst, float(np.max(np.abs(acc.astype(np.float64) - f((j + ss.astype(np.float64)) * h)))))
+    return worst
+
+
 def emit(name, coefs, comment):
     print("// %s" % comment)
     print("static constexpr int %s_N = %d;" % (name, len(coefs)))
@@ -149,6 +173,19 @@ def main():
     mD = cheb_monomial(D, 0.0, 64.0, 8)
     eD = check32(mD, D, 0.0, 64.0)
     emit("BESSEL_D", mD, "J3(x)/x^3 = sum c_k t^k, t = x^2/32 - 1, x in [0,8]; fp32 max abs err %.2e" % eD)
+    tb = piecewise_cubic(B, 65)
+    tc = piecewise_cubic(C, 65)
+    eb = check_piecewise32(tb, B)
+    ec = check_piecewise32(tc, C)
+    tab = np.concatenate([tb, tc], axis=1).ravel()
+    print("// B and C on y in [j, j+1), j = 0..64: c0 + s(c1 + s(c2 + s c3)), s = y - j;")
+    print("// 8 floats per interval (B then C); fp32 max abs err B %.2e, C %.2e on [0, 64]" % (eb, ec))
+    print("static constexpr int BESSEL_TAB_N = 65;")
+    print("__device__ static constexpr float BESSEL_TAB[%d] = {" % tab.size)
+    for j in range(65):
+        print("    " + " ".join("%.9ef," % v for v in tab[8 * j:8 * j + 8]))
+    print("};")
+    print()
     for n in (1, 2):
         P, Q = hankel_terms(n, 11)
         emit("HANKEL_P%d" % n, P, "P_%d(x) = sum a_k (8x)^(-2k)  (A&S 9.2.9)" % n)

@@ -144,6 +144,33 @@ __device__ inline void bessel_BC(float y, float* B, float* C) {
   else bessel_BC_asym(y, B, C);
 }
 
+// B and C for y <= 64 from polynomials (no table needed).
+struct BesselPoly {
+  __device__ void operator()(float y, float* B, float* C) const { bessel_BC_poly(y, B, C); }
+};
+
+// B and C for y in [0, 64] from the piecewise cubics of BESSEL_TAB, staged in
+// LDS by the kernel (tab = 2 float4 per unit interval of y: B, then C
+// coefficients).  Interval j = trunc(y), s = y - j.
+struct BesselTab {
+  const float4* tab;
+  __device__ void operator()(float y, float* B, float* C) const {
+    const int j = (int)y;
+    const float s = y - (float)j;
+    const float4 b = tab[2 * j], c = tab[2 * j + 1];
+    *B = fmaf(fmaf(fmaf(b.w, s, b.z), s, b.y), s, b.x);
+    *C = fmaf(fmaf(fmaf(c.w, s, c.z), s, c.y), s, c.x);
+  }
+};
+
+// Copies BESSEL_TAB into the work-group's LDS table (all threads take part;
+// call before any thread of the group leaves).
+__device__ inline void load_bessel_tab(float4* tab) {
+  float* t = reinterpret_cast<float*>(tab);
+  for (int i = (int)threadIdx.x; i < 8 * BESSEL_TAB_N; i += (int)blockDim.x) t[i] = BESSEL_TAB[i];
+  __syncthreads();
+}
+
 // ---------------------------------------------------------------- wave64
 
 __device__ inline int lane_id() { return (int)(threadIdx.x & 63u); }

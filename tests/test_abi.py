@@ -92,3 +92,28 @@ def test_validation_errors_are_codes_not_exits():
     f.idat[0] = 0
     rc, msg = _create([f])
     assert rc == -1 and "idat" in msg
+
+
+def _prototypes():
+    """name -> number of parameters, from the prototypes of include/mhre.h."""
+    text = re.sub(r"/\*.*?\*/", "", open(HEADER).read(), flags=re.S)
+    out = {}
+    for m in re.finditer(r"^\s*(?:const\s+)?(?:int|void|char)\s*\*?\s*(\w+)\s*\(([^)]*)\)\s*;", text, flags=re.M):
+        args = m.group(2).strip()
+        out[m.group(1)] = 0 if args in ("", "void") else len(args.split(","))
+    return out
+
+
+def test_ctypes_declarations_match_the_header():
+    """Every argtypes list the Python layer declares has the header's arity."""
+    L = _lib.load(LIBS[0])
+    protos = _prototypes()
+    checked = 0
+    for name, n in protos.items():
+        f = getattr(L, name)
+        if f.argtypes is None:
+            continue
+        assert len(f.argtypes) == n, "%s: ctypes declares %d arguments, header %d" % (name, len(f.argtypes), n)
+        checked += 1
+    assert checked >= 15
+    assert protos["mcmc_cuda"] == 44
