@@ -145,9 +145,12 @@ def batched_vs_plain(iters, nfits, log_every=16):
     plain = plain_vis / tp
     # the same fits one at a time through the reference's per-datum model()
     # contract (generic model.cuh path, one lane per chain)
+    log("  one session per fit done; now the model.cuh path")
     tg = 0.0
     gen_vis = 0.0
     for f, (d, nb, ns) in enumerate(fits):
+        if f % log_every == 0:
+            log("  model.cuh path: fit %d/%d" % (f, nfits))
         with Session([mk(d, nb, ns, f)], force_generic=True) as s:
             s.run(1)
             s.sync()
@@ -228,13 +231,19 @@ def quadsys_leg(nsys=1024, nburn=3000, niter=10000):
     ds = synthdata.quadsys_dataset(nsys=nsys, seed=5)
     fits = [quadsys.quad_fit(ds["Q"][s], ds["b"][s], ds["c"][s], nbeta=16, nseq=32, nburn=nburn, niter=niter,
                              seed=7000 + s, record=False) for s in range(nsys)]
+    log("  %d systems packed; sampling" % nsys)
     with Session(fits) as s:
         s.run(1)
         s.sync()
         e0 = s.stats()
         t0 = time.time()
-        s.run(nburn + niter - 1)
-        s.sync()
+        left = nburn + niter - 1
+        while left > 0:
+            n = min(1000, left)
+            s.run(n)
+            s.sync()
+            left -= n
+            log("  %d iterations left (%.1f s)" % (left, time.time() - t0))
         dt = time.time() - t0
         e1 = s.stats()
         s.download()

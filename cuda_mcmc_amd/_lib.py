@@ -84,6 +84,21 @@ def lib_path(path=None):
     return DEFAULT_LIB
 
 
+def _bring_up_torch_runtime():
+    """Load torch's HIP runtime before the engine library, when torch exists.
+
+    torch ships its own libamdhip64 with the same soname as the system one
+    the engine links.  Whichever is loaded first serves the whole process,
+    and torch's GPU support does not start on a runtime other than its own,
+    so the engine binds to torch's (the gradient interface hands torch
+    device buffers to the engine).  Without torch this does nothing.
+    """
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def load(path=None):
     """Load (once) and return the engine library."""
     p = lib_path(path)
@@ -93,6 +108,7 @@ def load(path=None):
         if not os.path.isfile(p):
             raise RuntimeError("MH-RE engine library not found at %s; build it with "
                                "`python -m cuda_mcmc_amd.build`" % p)
+        _bring_up_torch_runtime()
         L = C.CDLL(p)
         _declare(L)
         _libs[p] = L

@@ -85,29 +85,41 @@ struct Sgra9 {
   };
 
   // p[i] = parameter i (Zsp, Re, rq, ecc, fade, gax, aq, gq, th).
-  __device__ static Inv prepare(const float* p) {
-    Inv q;
-    const double Zsp = p[0], Re = p[1], rq = p[2], ecc = p[3], fade = p[4];
-    const double gax = p[5], aq = p[6], gq = p[7], th = p[8];
+  // The invariants in four groups by the parameters they depend on.
+  __device__ static void ring(Inv& q, const float* p) {   // Re, rq, ecc, fade
+    const double Re = p[1], rq = p[2], ecc = p[3], fade = p[4];
     const double Rex = Re * UAS2NRAD;
     const double Rin = rq * Rex;
     const double d = ecc * (Rex - Rin);
     ring_coefs(Rex, rq, fade, d, &q.r);
-    double s, c;
-    sincos(th, &s, &c);
-    q.cth = (float)c;
-    q.sth = (float)s;
-    const bool nog = (p[5] == 0.0f) || (p[6] == 0.0f) || (p[7] == 0.0f);
-    const double gsy = gax * Rex, gsx = aq * gsy;
+    q.kg = (float)(-2.0 * D_PI * (Rin - d));
+  }
+  __device__ static void rotation(Inv& q, const float* p) {   // th
+    q.cth = (float)cos((double)p[8]);
+    q.sth = (float)sin((double)p[8]);
+  }
+  __device__ static void gaussian(Inv& q, const float* p) {   // gax, aq (and Re)
+    const double Rex = (double)p[1] * UAS2NRAD;
+    const double gsy = (double)p[5] * Rex, gsx = (double)p[6] * gsy;
     q.gaU = (float)(GCOEF * LOG2E * gsx * gsx);
     q.gaV = (float)(GCOEF * LOG2E * gsy * gsy);
-    q.kg = (float)(-2.0 * D_PI * (Rin - d));
+  }
+  __device__ static void mixing(Inv& q, const float* p) {   // Zsp, gq (and gax, aq: Gaussian present)
+    const double Zsp = p[0], gq = p[7];
+    const bool nog = (p[5] == 0.0f) || (p[6] == 0.0f) || (p[7] == 0.0f);
     q.r.gauss = nog ? 0 : 1;
     q.r.cr = (float)(nog ? Zsp : Zsp * (1.0 - gq));
     q.r.cg = (float)(nog ? 0.0 : Zsp * gq);
-    return q;
   }
 
+  __device__ static Inv prepare(const float* p) {
+    Inv q;
+    ring(q, p);
+    rotation(q, p);
+    gaussian(q, p);
+    mixing(q, p);
+    return q;
+  }
 
   template <class BC = BesselPoly>
   __device__ static float2 vis(const Inv& q, float u, float v, bool any_large, const BC& bc = BC()) {
@@ -142,34 +154,7 @@ struct Sgra13 {
     int gonly;            // gq == 1: Gaussian only
   };
 
-  // p: Zsp, Re, rq, ecc, fade, gr, gax, aq, gq, alpha, beta, eta, th
-  __device__ static Inv prepare(const float* p) {
-    Inv q;
-    const double Zsp = p[0], Re = p[1], rq = p[2], ecc = p[3], fade = p[4];
-    const double gr = p[5], gax = p[6], aq = p[7], gq = p[8];
-    const double alpha = p[9], beta = p[10], eta = p[11], th = p[12];
-    const double Rex = Re * UAS2NRAD;
-    const double Rin = rq * Rex;
-    const double d = ecc * (Rex - Rin) * cos(eta - th);
-    ring_coefs(Rex, rq, fade, d, &q.r);
-    q.cth = (float)cos(th);
-    q.sth = (float)sin(th);
-    q.cb = (float)cos(beta);
-    q.sb = (float)sin(beta);
-    const bool gauss = (p[6] != 0.0f) && (p[7] != 0.0f) && (p[8] != 0.0f);
-    const double gsy = gax * Rex, gsx = aq * gsy;
-    q.gaU = (float)(GCOEF * LOG2E * gsx * gsx);
-    q.gaV = (float)(GCOEF * LOG2E * gsy * gsy);
-    q.kgu = (float)(-2.0 * D_PI * gr * Rex * cos(alpha));
-    q.kgv = (float)(-2.0 * D_PI * gr * Rex * sin(alpha));
-    q.gonly = (p[8] == 1.0f) ? 1 : 0;
-    q.r.gauss = gauss ? 1 : 0;
-    q.r.cr = (float)(gauss ? Zsp * (1.0 - gq) : Zsp);
-    q.r.cg = (float)(gauss ? Zsp * gq : 0.0);
-    if (q.gonly) { q.r.cr = 0.0f; q.r.cg = (float)(gauss ? Zsp : 0.0); }
-    return q;
-  }
-
+  __device__ static Inv prepare(const float* p);
 
   template <class BC = BesselPoly>
   __device__ static float2 vis(const Inv& q, float u, float v, bool any_large, const BC& bc = BC()) {
@@ -194,5 +179,33 @@ struct Sgra13 {
     return out;
   }
 };
+
+// p: Zsp, Re, rq, ecc, fade, gr, gax, aq, gq, alpha, beta, eta, th
+__device__ inline Sgra13::Inv Sgra13::prepare(const float* p) {
+  Inv q;
+  const double Zsp = p[0], Re = p[1], rq = p[2], ecc = p[3], fade = p[4];
+  const double gr = p[5], gax = p[6], aq = p[7], gq = p[8];
+  const double alpha = p[9], beta = p[10], eta = p[11], th = p[12];
+  const double Rex = Re * UAS2NRAD;
+  const double Rin = rq * Rex;
+  const double d = ecc * (Rex - Rin) * cos(eta - th);
+  ring_coefs(Rex, rq, fade, d, &q.r);
+  q.cth = (float)cos(th);
+  q.sth = (float)sin(th);
+  q.cb = (float)cos(beta);
+  q.sb = (float)sin(beta);
+  const bool gauss = (p[6] != 0.0f) && (p[7] != 0.0f) && (p[8] != 0.0f);
+  const double gsy = gax * Rex, gsx = aq * gsy;
+  q.gaU = (float)(GCOEF * LOG2E * gsx * gsx);
+  q.gaV = (float)(GCOEF * LOG2E * gsy * gsy);
+  q.kgu = (float)(-2.0 * D_PI * gr * Rex * cos(alpha));
+  q.kgv = (float)(-2.0 * D_PI * gr * Rex * sin(alpha));
+  q.gonly = (p[8] == 1.0f) ? 1 : 0;
+  q.r.gauss = gauss ? 1 : 0;
+  q.r.cr = (float)(gauss ? Zsp * (1.0 - gq) : Zsp);
+  q.r.cg = (float)(gauss ? Zsp * gq : 0.0);
+  if (q.gonly) { q.r.cr = 0.0f; q.r.cg = (float)(gauss ? Zsp : 0.0); }
+  return q;
+}
 
 }  // namespace mhre
