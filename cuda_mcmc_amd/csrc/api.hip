@@ -378,15 +378,18 @@ struct Session {
 template <class T>
 T* at(Session* S, size_t off) { return reinterpret_cast<T*>(S->arena + off); }
 
+// force_generic (or MHRE_FORCE_GENERIC) sends the ring+Gaussian models
+// through the compiled-in model.cuh as well.
 int resolve_kind(const MhreFitSpec& s, bool force_generic) {
+  const bool generic = force_generic || env_flag("MHRE_FORCE_GENERIC");
   switch (s.model) {
-    case MHRE_MODEL_SGRA9: return K_SGRA9;
-    case MHRE_MODEL_SGRA13: return K_SGRA13;
+    case MHRE_MODEL_SGRA9: return generic ? K_USER : K_SGRA9;
+    case MHRE_MODEL_SGRA13: return generic ? K_USER : K_SGRA13;
     case MHRE_MODEL_USER: return K_USER;
     case MHRE_MODEL_QUADSYS: return K_QUAD;
     default: break;
   }
-  if (kNativeEht && !force_generic && !env_flag("MHRE_FORCE_GENERIC")) {
+  if (kNativeEht && !generic) {
     if (s.imodel == 1) return K_SGRA9;
     if (s.imodel == 2) return K_SGRA13;
   }
@@ -607,6 +610,9 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
     if (!eval_only) CHK(validate(h.s, i));
     if (h.s.it0 != it0) FAIL(MHRE_EINVAL, "fit %d: all fits of a session must share it0", i);
     h.kind = resolve_kind(h.s, S->opt.force_generic != 0);
+    if (h.kind == K_USER && (h.s.model == MHRE_MODEL_SGRA9 || h.s.model == MHRE_MODEL_SGRA13) && !kNativeEht)
+      FAIL(MHRE_EINVAL, "fit %d: the ring+Gaussian models on the model.cuh path need a library built with "
+                        "examples/sgra_model_fit/model.cuh", i);
     h.nchain = h.s.nbeta * h.s.nseq;
     if (h.kind == K_SGRA9 || h.kind == K_SGRA13) CHK(pack_eht(h, i));
     if (h.kind == K_QUAD) CHK(pack_quad(h, i));
@@ -791,9 +797,13 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
     KindWork kw;
     kw.kind = kk;
     kw.qslot = qslot++;
-    for (int i : idx)
+    for (int i : idx) {
       for (int c = 0; c < S->fits[i].nchain; ++c) { kw.step.push_back({i, c}); kw.eval.push_back({i, c}); }
-    kw.xr = deal_xcd(kw.step, fit_cost);
+      // lane-per-chain kernels: every wave holds chains of one fit
+      if (kk == K_USER)
+        while (kw.step.size() % 64) kw.step.push_back({i, -1});
+    }
+    if (kk != K_USER) kw.xr = deal_xcd(kw.step, fit_cost);
     S->kinds.push_back(std::move(kw));
   }
   std::vector<double> seq_cost(nfits);

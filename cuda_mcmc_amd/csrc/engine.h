@@ -2,8 +2,24 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include "fitdesc.h"
+#include "mhre_device.h"
 
 namespace mhre {
+
+// Adds the wave's performed evaluations and proposals to the fit's counters
+// with one atomic each (every lane of the wave, padding included, calls it).
+__device__ inline void wave_count(const FitDesc& f, unsigned nev, unsigned nprop) {
+  unsigned a = nev, b = nprop;
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) {
+    a += (unsigned)__shfl_xor((int)a, m, 64);
+    b += (unsigned)__shfl_xor((int)b, m, 64);
+  }
+  if (f.nevals && lane_id() == 0) {
+    atomicAdd(&f.nevals[0], (unsigned long long)a);
+    atomicAdd(&f.nevals[1], (unsigned long long)b);
+  }
+}
 
 // Optional record of the decisions of iterations it0 .. it0+nit-1.
 struct TraceBuf {

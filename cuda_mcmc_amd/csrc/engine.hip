@@ -475,19 +475,11 @@ __global__ void __launch_bounds__(256) k_eval_quad(const FitDesc* fits, const Wo
   f.chi2c[w.idx] = (float)QuadSys<NQ>::chi2(f.qcoef, f.quad_neq, x);
 }
 
-// One lane per chain; all chains of a wave belong to one fit (the host pads
-// every fit's chain list to whole waves), so the coefficients are uniform.
+// The chain's nprm proposals of this iteration (one lane); returns the
+// chi^2 evaluations performed.
 template <int NQ>
-__global__ void __launch_bounds__(256) k_step_quad(const FitDesc* fits, const WorkItem* work, int nwork,
-                                                   RunClock* clk, TraceBuf tr) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= nwork) return;
-  const WorkItem w = work[g];
-  if (w.idx < 0) return;   // padding lane
-  const FitDesc& f = fits[w.fit];
-  const int it = load_clock(clk);
-  if (it >= f.nburn + f.niter) return;
-  const int ch = w.idx, C = f.nchain, P = f.nprm;
+__device__ unsigned quad_one_chain(const FitDesc& f, int ch, int it, const TraceBuf& tr) {
+  const int C = f.nchain, P = f.nprm;
   const int ibeta = ch / f.nseq;
   const bool burn = it < f.nburn;
   float x[NQ];
@@ -558,10 +550,26 @@ __global__ void __launch_bounds__(256) k_step_quad(const FitDesc* fits, const Wo
   f.flag[ch] = flag;
   f.ptent[ch] = told;
   f.ptentn[ch] = tlast;
-  if (f.nevals) {
-    atomicAdd(&f.nevals[0], (unsigned long long)nev);
-    atomicAdd(&f.nevals[1], (unsigned long long)P);
-  }
+  return nev;
+}
+
+
+// One lane per chain; all chains of a wave belong to one fit (the host pads
+// every fit's chain list to whole waves), so the coefficients are uniform.
+template <int NQ>
+__global__ void __launch_bounds__(256) k_step_quad(const FitDesc* fits, const WorkItem* work, int nwork,
+                                                   RunClock* clk, TraceBuf tr) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (__builtin_amdgcn_readfirstlane(g) >= nwork) return;   // whole waves (the list is padded to 64)
+  const WorkItem w = work[g];
+  // every lane of a wave belongs to the same fit: its coefficients and sizes
+  // are read through scalar loads
+  const FitDesc& f = fits[__builtin_amdgcn_readfirstlane(w.fit)];
+  const int it = __builtin_amdgcn_readfirstlane(load_clock(clk));
+  if (it >= f.nburn + f.niter) return;
+  const bool live = w.idx >= 0;   // padding lanes only take part in the counting
+  const unsigned nev = live ? quad_one_chain<NQ>(f, w.idx, it, tr) : 0u;
+  wave_count(f, nev, live ? (unsigned)f.nprm : 0u);
 }
 
 // ---------------------------------------------------------------- exchange

@@ -72,15 +72,10 @@ __global__ void __launch_bounds__(64) k_eval_user(const FitDesc* fits, const Wor
   if (err && f.errflag) atomicExch(f.errflag, err);
 }
 
-__global__ void __launch_bounds__(64) k_step_user(const FitDesc* fits, const WorkItem* work, int nwork,
-                                                  RunClock* clk, TraceBuf tr) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= nwork) return;
-  const WorkItem w = work[g];
-  const FitDesc& f = fits[w.fit];
-  const int it = __atomic_load_n(&clk->it, __ATOMIC_RELAXED);
-  if (it >= f.nburn + f.niter) return;
-  const int ch = w.idx, C = f.nchain, P = f.nprm, N = f.nptot;
+// The chain's nprm proposals of this iteration (one lane); returns the
+// chi^2 evaluations performed.
+__device__ unsigned user_one_chain(const FitDesc& f, int ch, int it, const TraceBuf& tr) {
+  const int C = f.nchain, P = f.nprm, N = f.nptot;
   const int ibeta = ch / f.nseq;
   const bool burn = it < f.nburn;
   const double beta = (double)f.beta[ibeta];
@@ -146,11 +141,23 @@ __global__ void __launch_bounds__(64) k_step_user(const FitDesc* fits, const Wor
   f.flag[ch] = flag;
   f.ptent[ch] = told;
   f.ptentn[ch] = tlast;
-  if (f.nevals) {
-    atomicAdd(&f.nevals[0], (unsigned long long)nev);
-    atomicAdd(&f.nevals[1], (unsigned long long)P);
-  }
   if (err && f.errflag) atomicExch(f.errflag, err);
+  return nev;
+}
+
+// One lane per chain; every fit's chain list is padded to whole waves, so a
+// wave's lanes share one fit and its counters take one atomic per wave.
+__global__ void __launch_bounds__(64) k_step_user(const FitDesc* fits, const WorkItem* work, int nwork,
+                                                  RunClock* clk, TraceBuf tr) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (__builtin_amdgcn_readfirstlane(g) >= nwork) return;
+  const WorkItem w = work[g];
+  const FitDesc& f = fits[__builtin_amdgcn_readfirstlane(w.fit)];
+  const int it = __builtin_amdgcn_readfirstlane(__atomic_load_n(&clk->it, __ATOMIC_RELAXED));
+  if (it >= f.nburn + f.niter) return;
+  const bool live = w.idx >= 0;
+  const unsigned nev = live ? user_one_chain(f, w.idx, it, tr) : 0u;
+  wave_count(f, nev, live ? (unsigned)f.nprm : 0u);
 }
 
 static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
