@@ -83,12 +83,23 @@ def build_library(model_dir, out, native_eht=False, tag=None):
     return out
 
 
+def build_pmc_copy(verbose=True):
+    """The HBM byte-counter calibration program (cuda_mcmc_amd/lib/pmc_copy)."""
+    out = os.path.join(LIBDIR, "pmc_copy")
+    if verbose:
+        print("building", os.path.relpath(out, ROOT), flush=True)
+    os.makedirs(LIBDIR, exist_ok=True)
+    _run([HIPCC] + FLAGS + [os.path.join(CSRC, "pmc_copy.hip"), "-o", out])
+    return out
+
+
 def build_all(verbose=True):
     built = []
     for model_dir, out, native in IN_TREE:
         if verbose:
             print("building", os.path.relpath(out, ROOT), flush=True)
         built.append(build_library(model_dir, out, native))
+    built.append(build_pmc_copy(verbose))
     return built
 
 

@@ -653,7 +653,15 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
     h.o_tout = A.reserve<int>(C * nrec);
     h.o_chi2 = A.reserve<float>((size_t)s.nseq * nrec);
     h.has_datm = h.kind == K_USER || s.datm != nullptr;
-    const size_t ndatm = h.has_datm ? std::max<size_t>((size_t)std::max(s.ndatm, 0), C * s.ndat) : 0;
+    // the model data head (C * ndat) and, for the ring+Gaussian model.cuh,
+    // its phase tail (C * nvis, src/imgpu.py:770-773)
+    size_t need = C * (size_t)s.ndat;
+    if (h.kind == K_USER && kNativeEht && (s.imodel == 1 || s.imodel == 2) && s.idat && s.nidat >= 1)
+      need += C * (size_t)std::max(s.idat[0], 0);
+    const size_t ndatm = h.has_datm ? std::max<size_t>((size_t)std::max(s.ndatm, 0), need) : 0;
+    if (h.kind == K_USER && ndatm > (size_t)0x7fffffff)
+      FAIL(MHRE_EINVAL, "fit %d: the model.cuh contract indexes datm with int; nbeta*nseq*(ndat + tail) = %zu "
+                        "exceeds 2^31 - 1", i, ndatm);
     h.o_datm = A.reserve<float>(ndatm);
     h.o_chi2m = A.reserve<float>(h.has_datm ? C * s.ndat : 0);
     h.o_mc = A.reserve<CCalcModel>(1);

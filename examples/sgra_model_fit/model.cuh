@@ -26,17 +26,20 @@ __device__ int model(CCalcModel *mc, int id, int ipt, int imd, int ipass) {
   if (mc->imodel != 1 && mc->imodel != 2) return -1;
   const int nvis = mc->idat[0];
   const int ist = imd / mc->ndat;
+  // the phase tail of this chain must lie inside datm[ndatm]
+  const bool tail = (long long)mc->nchi2m + (long long)(ist + 1) * nvis <= (long long)mc->ndatm;
   float *phase = mc->datm + mc->nchi2m + ist * nvis;
   if (ipass == 0 && id < nvis) {
     const float2 V = mhre_user_vis(mc, ipt, mc->coor[2 * id], mc->coor[2 * id + 1]);
     mc->datm[imd] = sqrtf(V.x * V.x + V.y * V.y);
-    phase[id] = atan2f(V.y, V.x);
+    if (tail) phase[id] = atan2f(V.y, V.x);
     return 1;
   }
   if (ipass == 1 && id >= nvis) {
     const int j = id - nvis;
     float cph;
     if (mc->idat[2] == 0) {
+      if (!tail) return -1;   // closures need the stored phases
       cph = phase[mc->icoor[3 * j]] + phase[mc->icoor[3 * j + 1]] - phase[mc->icoor[3 * j + 2]];
     } else {
       const float *c = mc->coor + 2 * nvis + 4 * j;
