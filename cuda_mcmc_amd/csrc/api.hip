@@ -439,6 +439,104 @@ int d2h(Session* S, T* dst, size_t off, size_t n) {
     if (r_ != MHRE_OK) return r_; \
   } while (0)
 
+// Many small arena transfers (a session of a thousand small fits has tens of
+// thousands of arrays) gathered into a few large copies.  Ranges closer than
+// MERGE_GAP bytes travel in one copy; ranges of at least DIRECT bytes are
+// copied straight between the arena and the caller's memory.
+constexpr size_t MERGE_GAP = 64 << 10;
+constexpr size_t DIRECT = 4 << 20;
+
+struct Range {
+  size_t off, bytes;
+  unsigned char* host;   // destination (download) or copied source (upload)
+};
+
+// Sorted, non-overlapping ranges -> runs [first, last) whose gaps are < MERGE_GAP.
+static std::vector<std::pair<size_t, size_t>> merge_runs(std::vector<Range>& r) {
+  std::sort(r.begin(), r.end(), [](const Range& a, const Range& b) { return a.off < b.off; });
+  std::vector<std::pair<size_t, size_t>> runs;
+  for (size_t i = 0; i < r.size();) {
+    size_t j = i + 1, end = r[i].off + r[i].bytes;
+    while (j < r.size() && r[j].off < end + MERGE_GAP) {
+      end = std::max(end, r[j].off + r[j].bytes);
+      ++j;
+    }
+    runs.push_back({i, j});
+    i = j;
+  }
+  return runs;
+}
+
+// Uploads: each source is copied into host memory when added, so callers may
+// reuse their buffers at once; flush() copies the runs and waits for them.
+struct Uploads {
+  std::vector<Range> r;
+  std::vector<std::vector<unsigned char>> keep;
+  template <class T>
+  void add(size_t off, const T* src, size_t n) {
+    if (!src || n == 0) return;
+    keep.emplace_back((const unsigned char*)src, (const unsigned char*)src + n * sizeof(T));
+    r.push_back({off, n * sizeof(T), keep.back().data()});
+  }
+  int flush(Session* S) {
+    std::vector<std::vector<unsigned char>> stage;
+    for (auto [a, b] : merge_runs(r)) {
+      const size_t lo = r[a].off, hi = r[b - 1].off + r[b - 1].bytes;
+      if (b - a == 1) {
+        HIPCHK(hipMemcpyAsync(S->arena + lo, r[a].host, r[a].bytes, hipMemcpyHostToDevice, S->stream));
+        continue;
+      }
+      // gap bytes are arena padding or arrays the kernels write before reading
+      stage.emplace_back(hi - lo, 0);
+      for (size_t i = a; i < b; ++i) std::memcpy(stage.back().data() + (r[i].off - lo), r[i].host, r[i].bytes);
+      HIPCHK(hipMemcpyAsync(S->arena + lo, stage.back().data(), hi - lo, hipMemcpyHostToDevice, S->stream));
+    }
+    HIPCHK(hipStreamSynchronize(S->stream));   // the host copies must outlive the transfers
+    r.clear();
+    keep.clear();
+    return MHRE_OK;
+  }
+};
+
+// Downloads: flush() copies the runs into a host staging buffer (large ranges
+// straight to their destination) and scatters them.
+struct Downloads {
+  std::vector<Range> r;
+  template <class T>
+  void add(T* dst, size_t off, size_t n) {
+    if (!dst || n == 0) return;
+    r.push_back({off, n * sizeof(T), (unsigned char*)dst});
+  }
+  int flush(Session* S) {
+    std::vector<Range> small;
+    for (const Range& x : r) {
+      if (x.bytes >= DIRECT)
+        HIPCHK(hipMemcpyAsync(x.host, S->arena + x.off, x.bytes, hipMemcpyDeviceToHost, S->stream));
+      else
+        small.push_back(x);
+    }
+    const auto runs = merge_runs(small);
+    size_t total = 0;
+    for (auto [a, b] : runs) total += small[b - 1].off + small[b - 1].bytes - small[a].off;
+    std::vector<unsigned char> stage(total);
+    size_t at = 0;
+    for (auto [a, b] : runs) {
+      const size_t lo = small[a].off, hi = small[b - 1].off + small[b - 1].bytes;
+      HIPCHK(hipMemcpyAsync(stage.data() + at, S->arena + lo, hi - lo, hipMemcpyDeviceToHost, S->stream));
+      at += hi - lo;
+    }
+    HIPCHK(hipStreamSynchronize(S->stream));
+    at = 0;
+    for (auto [a, b] : runs) {
+      const size_t lo = small[a].off;
+      for (size_t i = a; i < b; ++i) std::memcpy(small[i].host, stage.data() + at + (small[i].off - lo), small[i].bytes);
+      at += small[b - 1].off + small[b - 1].bytes - lo;
+    }
+    r.clear();
+    return MHRE_OK;
+  }
+};
+
 StepLaunch make_launch(Session* S, const KindWork& k) {
   StepLaunch L{};
   L.fits = S->d_fits;
@@ -681,9 +779,11 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
   HIPCHK(hipMemsetAsync(S->d_err, 0, sizeof(int), S->stream));
   RunClock clk{};
   clk.it = it0;
+  clk.xit = it0;
   HIPCHK(hipMemcpyAsync(S->d_clk, &clk, sizeof(clk), hipMemcpyHostToDevice, S->stream));
 
   std::vector<FitDesc> descs(nfits);
+  Uploads U;
   for (int i = 0; i < nfits; ++i) {
     FitHost& h = S->fits[i];
     const MhreFitSpec& s = h.s;
@@ -722,42 +822,42 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
     d.errflag = S->d_err;
     descs[i] = d;
 
-    CHK(h2d(S, h.o_vis, h.vis.data(), h.vis.size()));
-    CHK(h2d(S, h.o_clo, h.clo.data(), h.clo.size()));
-    CHK(h2d(S, h.o_clow, h.clow.data(), h.clow.size()));
-    CHK(h2d(S, h.o_tinfo, h.tinfo.data(), h.tinfo.size()));
-    CHK(h2d(S, h.o_qcoef, h.qcoef.data(), h.qcoef.size()));
-    CHK(h2d(S, h.o_coor, s.coor, s.coor ? s.ncoor : 0));
-    CHK(h2d(S, h.o_dat, s.dat, s.ndat));
-    CHK(h2d(S, h.o_std2r, s.std2r, s.ndat));
-    CHK(h2d(S, h.o_icoor, s.icoor, s.icoor ? s.nicoor : 0));
-    CHK(h2d(S, h.o_idat, s.idat, s.idat ? s.nidat : 0));
-    CHK(h2d(S, h.o_pdescr, s.pdescr, N));
-    CHK(h2d(S, h.o_ivar, s.ivar, P));
-    CHK(h2d(S, h.o_invar, s.invar, N));
-    CHK(h2d(S, h.o_pmint, s.pmint, N));
-    CHK(h2d(S, h.o_pmaxt, s.pmaxt, N));
-    CHK(h2d(S, h.o_beta, s.beta, s.nbeta));
-    CHK(h2d(S, h.o_pcur, s.pcur, C * N));
-    CHK(h2d(S, h.o_chi2c, s.chi2c, C));
-    CHK(h2d(S, h.o_pstp, s.pstp, P * C));
-    CHK(h2d(S, h.o_ncnt, s.n_cnt, P * C));
-    CHK(h2d(S, h.o_nacpt, s.n_acpt, P * C));
+    U.add(h.o_vis, h.vis.data(), h.vis.size());
+    U.add(h.o_clo, h.clo.data(), h.clo.size());
+    U.add(h.o_clow, h.clow.data(), h.clow.size());
+    U.add(h.o_tinfo, h.tinfo.data(), h.tinfo.size());
+    U.add(h.o_qcoef, h.qcoef.data(), h.qcoef.size());
+    U.add(h.o_coor, s.coor, s.coor ? s.ncoor : 0);
+    U.add(h.o_dat, s.dat, s.ndat);
+    U.add(h.o_std2r, s.std2r, s.ndat);
+    U.add(h.o_icoor, s.icoor, s.icoor ? s.nicoor : 0);
+    U.add(h.o_idat, s.idat, s.idat ? s.nidat : 0);
+    U.add(h.o_pdescr, s.pdescr, N);
+    U.add(h.o_ivar, s.ivar, P);
+    U.add(h.o_invar, s.invar, N);
+    U.add(h.o_pmint, s.pmint, N);
+    U.add(h.o_pmaxt, s.pmaxt, N);
+    U.add(h.o_beta, s.beta, s.nbeta);
+    U.add(h.o_pcur, s.pcur, C * N);
+    U.add(h.o_chi2c, s.chi2c, C);
+    U.add(h.o_pstp, s.pstp, P * C);
+    U.add(h.o_ncnt, s.n_cnt, P * C);
+    U.add(h.o_nacpt, s.n_acpt, P * C);
     pack_hist(s.n_hist, s.nadj, P * C, h.hbits);
-    CHK(h2d(S, h.o_nhist, h.hbits.data(), h.hbits.size()));
-    CHK(h2d(S, h.o_nexch, s.n_exch, C));
-    CHK(h2d(S, h.o_tcur, s.tcur, C));
-    CHK(h2d(S, h.o_flag, s.flag, C));
-    if (s.ptent) CHK(h2d(S, h.o_ptent, s.ptent, C));
-    if (s.ptentn) CHK(h2d(S, h.o_ptentn, s.ptentn, C));
-    if (h.has_datm && s.datm) CHK(h2d(S, h.o_datm, s.datm, (size_t)std::min<long long>(d.ndatm, s.ndatm)));
-    if (h.has_datm && s.chi2m) CHK(h2d(S, h.o_chi2m, s.chi2m, C * s.ndat));
+    U.add(h.o_nhist, h.hbits.data(), h.hbits.size());
+    U.add(h.o_nexch, s.n_exch, C);
+    U.add(h.o_tcur, s.tcur, C);
+    U.add(h.o_flag, s.flag, C);
+    if (s.ptent) U.add(h.o_ptent, s.ptent, C);
+    if (s.ptentn) U.add(h.o_ptentn, s.ptentn, C);
+    if (h.has_datm && s.datm) U.add(h.o_datm, s.datm, (size_t)std::min<long long>(d.ndatm, s.ndatm));
+    if (h.has_datm && s.chi2m) U.add(h.o_chi2m, s.chi2m, C * s.ndat);
     // running accept sums over the nadj-deep history (kept exactly equal to
     // sum_a n_hist[a][k][chain])
     h.sumh.assign(P * C, 0);
     for (int a = 0; a < s.nadj; ++a)
       for (size_t q = 0; q < P * C; ++q) h.sumh[q] += s.n_hist[(size_t)a * P * C + q];
-    CHK(h2d(S, h.o_sumh, h.sumh.data(), h.sumh.size()));
+    U.add(h.o_sumh, h.sumh.data(), h.sumh.size());
     CCalcModel mc{};
     mc.imodel = s.imodel; mc.ncoor = s.ncoor; mc.ndat = s.ndat; mc.nicoor = s.nicoor; mc.nidat = s.nidat;
     mc.ndatm = (int)std::min<long long>(d.ndatm, 0x7fffffff); mc.nptot = s.nptot; mc.nbeta = s.nbeta; mc.nseq = s.nseq;
@@ -765,9 +865,9 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
     mc.coor = const_cast<float*>(d.coor); mc.dat = const_cast<float*>(d.dat); mc.std2r = const_cast<float*>(d.std2r);
     mc.icoor = const_cast<int*>(d.icoor); mc.idat = const_cast<int*>(d.idat);
     mc.datm = d.datm; mc.chi2m = d.chi2m; mc.pcur = d.pcur; mc.flag = d.flag; mc.chi2c = d.chi2c;
-    HIPCHK(hipMemcpyAsync(S->arena + h.o_mc, &mc, sizeof(mc), hipMemcpyHostToDevice, S->stream));
-    HIPCHK(hipStreamSynchronize(S->stream));   // host staging vectors must outlive the copies
+    U.add(h.o_mc, &mc, 1);
   }
+  CHK(U.flush(S));
   HIPCHK(hipMemcpyAsync(S->d_fits, descs.data(), sizeof(FitDesc) * nfits, hipMemcpyHostToDevice, S->stream));
 
   // work lists: fits grouped per kind, heaviest fits first
@@ -926,28 +1026,29 @@ int run(Session* S, int niter) {
 
 int download(Session* S, int model_data) {
   HIPCHK(hipStreamSynchronize(S->stream));
+  Downloads D;
   for (FitHost& h : S->fits) {
     const MhreFitSpec& s = h.s;
     const size_t C = (size_t)h.nchain, P = (size_t)s.nprm, N = (size_t)s.nptot;
-    CHK(d2h(S, s.pcur, h.o_pcur, C * N));
-    CHK(d2h(S, s.chi2c, h.o_chi2c, C));
-    CHK(d2h(S, s.pstp, h.o_pstp, P * C));
-    CHK(d2h(S, s.n_cnt, h.o_ncnt, P * C));
-    CHK(d2h(S, s.n_acpt, h.o_nacpt, P * C));
+    D.add(s.pcur, h.o_pcur, C * N);
+    D.add(s.chi2c, h.o_chi2c, C);
+    D.add(s.pstp, h.o_pstp, P * C);
+    D.add(s.n_cnt, h.o_ncnt, P * C);
+    D.add(s.n_acpt, h.o_nacpt, P * C);
     h.hbits.resize((size_t)hist_words(s.nadj) * P * C);
-    CHK(d2h(S, h.hbits.data(), h.o_nhist, h.hbits.size()));
-    CHK(d2h(S, s.n_exch, h.o_nexch, C));
-    CHK(d2h(S, s.tcur, h.o_tcur, C));
-    CHK(d2h(S, s.flag, h.o_flag, C));
-    CHK(d2h(S, s.ptent, h.o_ptent, C));
-    CHK(d2h(S, s.ptentn, h.o_ptentn, C));
+    D.add(h.hbits.data(), h.o_nhist, h.hbits.size());
+    D.add(s.n_exch, h.o_nexch, C);
+    D.add(s.tcur, h.o_tcur, C);
+    D.add(s.flag, h.o_flag, C);
+    D.add(s.ptent, h.o_ptent, C);
+    D.add(s.ptentn, h.o_ptentn, C);
     if (h.record) {
-      CHK(d2h(S, s.pout, h.o_pout, P * C * (size_t)s.niter));
-      CHK(d2h(S, s.tout, h.o_tout, C * (size_t)s.niter));
-      CHK(d2h(S, s.chi2, h.o_chi2, (size_t)s.nseq * s.niter));
+      D.add(s.pout, h.o_pout, P * C * (size_t)s.niter);
+      D.add(s.tout, h.o_tout, C * (size_t)s.niter);
+      D.add(s.chi2, h.o_chi2, (size_t)s.nseq * s.niter);
     }
   }
-  HIPCHK(hipStreamSynchronize(S->stream));
+  CHK(D.flush(S));
   for (FitHost& h : S->fits)
     unpack_hist(h.hbits, h.s.nadj, (size_t)h.s.nprm * h.nchain, h.s.n_hist);
   if (model_data) {
@@ -960,9 +1061,10 @@ int download(Session* S, int model_data) {
       const MhreFitSpec& s = h.s;
       if (!h.has_datm) continue;
       const size_t C = (size_t)h.nchain;
-      if (s.datm) CHK(d2h(S, s.datm, h.o_datm, (size_t)std::min<long long>(h.d.ndatm, s.ndatm)));
-      if (s.chi2m) CHK(d2h(S, s.chi2m, h.o_chi2m, C * s.ndat));
+      if (s.datm) D.add(s.datm, h.o_datm, (size_t)std::min<long long>(h.d.ndatm, s.ndatm));
+      if (s.chi2m) D.add(s.chi2m, h.o_chi2m, C * s.ndat);
     }
+    CHK(D.flush(S));
   }
   // resume records in rndst: seed (8 B), next iteration (4 B), chain (4 B)
   int it = 0;

@@ -6,6 +6,27 @@
 
 namespace mhre {
 
+// The descriptor of fit `fit`, which must be the same on every lane of the
+// wave.  Descriptors are written by the host before a launch and by no
+// kernel, so the uniform address is taken through the constant address space
+// and every field is read with a scalar load; the returned plain reference
+// keeps that address space once the callers are inlined.
+__device__ inline const FitDesc& wave_fit(const FitDesc* fits, int fit) {
+  return *(const FitDesc*)wave_const(fits + __builtin_amdgcn_readfirstlane(fit));
+}
+
+// Step launches: the first thread of the grid hands the iteration on to the
+// exchange launch that follows (RunClock); every thread reads it with
+// load_clock.  Kept as two calls so that a kernel can do the hand-over
+// before its set-up (which, in the fused step kernel, keeps its register
+// allocation at 96 VGPRs).
+__device__ inline void pass_clock(RunClock* clk) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) clk->xit = clk->it;
+}
+__device__ inline int load_clock(const RunClock* clk) {
+  return __atomic_load_n(&clk->it, __ATOMIC_RELAXED);
+}
+
 // Adds the wave's performed evaluations and proposals to the fit's counters
 // with one atomic each (every lane of the wave, padding included, calls it).
 __device__ inline void wave_count(const FitDesc& f, unsigned nev, unsigned nprop) {

@@ -45,10 +45,6 @@ __device__ inline double normal64(uint32_t w1, uint32_t w2) {
   return sqrt(-2.0 * log(u1)) * cos(6.283185307179586 * u2);
 }
 
-__device__ inline int load_clock(const RunClock* clk) {
-  return __atomic_load_n(&clk->it, __ATOMIC_RELAXED);
-}
-
 // ------------------------------------------------------------- init chains
 
 __global__ void k_init_chains(const FitDesc* fits, const WorkItem* work, int nwork) {
@@ -72,17 +68,6 @@ __global__ void k_init_chains(const FitDesc* fits, const WorkItem* work, int nwo
 
 // Wave-uniform view of one fit's native-kernel inputs; built once per wave
 // from a uniform FitDesc address so every field lives in scalar registers.
-// Global address space for the device pass (plain pointers in the host pass,
-// which never dereferences them): loads become global_load with scalar
-// bases instead of flat loads, which also count against the LDS counter.
-#if defined(__HIP_DEVICE_COMPILE__)
-#define GLOBAL __attribute__((address_space(1)))
-#else
-#define GLOBAL
-#endif
-template <class T>
-__device__ inline GLOBAL T* gptr(T* p) { return (GLOBAL T*)p; }
-
 // Element idx of a global array through a 32-bit byte offset from a
 // wave-uniform base (one global_load with a scalar base, no 64-bit address
 // arithmetic per lane).
@@ -90,12 +75,18 @@ template <class T>
 __device__ inline T ldg(const GLOBAL T* base, int idx) {
   return *(const GLOBAL T*)((const GLOBAL char*)base + (uint32_t)idx * (uint32_t)sizeof(T));
 }
+#if defined(__HIP_DEVICE_COMPILE__)   // the host pass has one address space
+template <class T>
+__device__ inline T ldg(const CONSTANT T* base, int idx) {
+  return base[idx];
+}
+#endif
 
 struct EhtView {
   const GLOBAL float4* vis;
   const GLOBAL int4* clo;
   const GLOBAL float4* clow;
-  const GLOBAL int2* tinfo;
+  const CONSTANT int2* tinfo;   // per-tile entries: read with scalar loads
   GLOBAL float* datm;
   GLOBAL float* chi2m;
   long long ndatm;
@@ -104,7 +95,7 @@ struct EhtView {
 
 __device__ inline EhtView eht_view(const FitDesc& f) {
   EhtView v;
-  v.vis = gptr(f.vis); v.clo = gptr(f.clo); v.clow = gptr(f.clow); v.tinfo = gptr(f.tinfo);
+  v.vis = gptr(f.vis); v.clo = gptr(f.clo); v.clow = gptr(f.clow); v.tinfo = wave_const(f.tinfo);
   v.datm = gptr(f.datm); v.chi2m = gptr(f.chi2m); v.ndatm = f.ndatm;
   v.nvis = f.nvis; v.ncph = f.ncph; v.ntile = f.ntile; v.ring_ok = f.ring_ok; v.cgroup = f.cgroup;
   v.ndat = f.ndat; v.nchain = f.nchain;
@@ -251,8 +242,8 @@ __global__ void __launch_bounds__(256) k_eval_eht(const FitDesc* fits, const Wor
   if (wv >= nwork) return;
   float2* ring = ring_all[wid];
   const WorkItem w = work[wv];
-  const FitDesc& f = fits[w.fit];
-  const int ch = w.idx;
+  const FitDesc& f = wave_fit(fits, w.fit);
+  const int ch = __builtin_amdgcn_readfirstlane(w.idx);
   const int lane = lane_id();
   if (respect_flag && f.flag[ch] == 0) {
     double s = 0.0;
@@ -405,6 +396,7 @@ __global__ void __launch_bounds__(64 * WPB) k_step_eht(const FitDesc* fits, cons
                                                        RunClock* clk, TraceBuf tr, int qslot) {
   __shared__ float2 ring_all[WPB][RING];
   __shared__ float4 btab[2 * BESSEL_TAB_N];
+  pass_clock(clk);
   load_bessel_tab(btab);
   const BesselTab bt{btab};
   const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -415,9 +407,9 @@ __global__ void __launch_bounds__(64 * WPB) k_step_eht(const FitDesc* fits, cons
     const int wv = xr.off[xb] + (int)(blockIdx.x / NXCD) * WPB + wid;
     if (wv >= xr.off[xb + 1]) return;
     const WorkItem w = work[wv];
-    const FitDesc& f = fits[w.fit];
+    const FitDesc& f = wave_fit(fits, w.fit);
     if (it >= f.nburn + f.niter) return;
-    step_one_chain<M>(f, w.idx, it, ring, bt, tr);
+    step_one_chain<M>(f, __builtin_amdgcn_readfirstlane(w.idx), it, ring, bt, tr);
   } else {
     for (int r = 0; r < NXCD; ++r) {
       const int x = (xb + r) % NXCD;
@@ -425,8 +417,8 @@ __global__ void __launch_bounds__(64 * WPB) k_step_eht(const FitDesc* fits, cons
       const int q = qslot * NXCD + x;
       for (int j = grab_item(clk, q); j < len; j = grab_item(clk, q)) {
         const WorkItem w = work[base + j];
-        const FitDesc& f = fits[w.fit];
-        if (it < f.nburn + f.niter) step_one_chain<M>(f, w.idx, it, ring, bt, tr);
+        const FitDesc& f = wave_fit(fits, w.fit);
+        if (it < f.nburn + f.niter) step_one_chain<M>(f, __builtin_amdgcn_readfirstlane(w.idx), it, ring, bt, tr);
       }
     }
   }
@@ -441,7 +433,10 @@ template <int NQ>
 struct QuadSys {
   static constexpr int NS = NQ * (NQ + 1) / 2;
   static constexpr int QS = NS + NQ + 2;
-  __device__ static double chi2(const float* __restrict__ qc, int neq, const float (&x)[NQ]) {
+  // qc: the fit's coefficients; the step kernel passes a wave-uniform
+  // constant-space pointer (scalar loads), the evaluation kernel a global one
+  template <class CP>
+  __device__ static double chi2(CP qc, int neq, const float (&x)[NQ]) {
     float xx[NS];
     int m = 0;
 #pragma unroll
@@ -450,12 +445,14 @@ struct QuadSys {
       for (int j = i; j < NQ; ++j) xx[m++] = x[i] * x[j];
     double acc = 0.0;
     for (int e = 0; e < neq; ++e) {
-      const float* c = qc + (size_t)e * QS;
-      float lhs = 0.0f;
+      const CP c = qc + (size_t)e * QS;
+      // four interleaved partial sums keep the dependent FMA chain short
+      float l[4] = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-      for (int i = 0; i < NS; ++i) lhs = fmaf(c[i], xx[i], lhs);
+      for (int i = 0; i < NS; ++i) l[i & 3] = fmaf(c[i], xx[i], l[i & 3]);
 #pragma unroll
-      for (int i = 0; i < NQ; ++i) lhs = fmaf(c[NS + i], x[i], lhs);
+      for (int i = 0; i < NQ; ++i) l[(NS + i) & 3] = fmaf(c[NS + i], x[i], l[(NS + i) & 3]);
+      const float lhs = (l[0] + l[1]) + (l[2] + l[3]);
       const float r = c[NS + NQ] - lhs;
       acc += (double)(r * r * c[NS + NQ + 1]);
     }
@@ -472,7 +469,7 @@ __global__ void __launch_bounds__(256) k_eval_quad(const FitDesc* fits, const Wo
   float x[NQ];
 #pragma unroll
   for (int i = 0; i < NQ; ++i) x[i] = f.pcur[(size_t)w.idx * NQ + i];
-  f.chi2c[w.idx] = (float)QuadSys<NQ>::chi2(f.qcoef, f.quad_neq, x);
+  f.chi2c[w.idx] = (float)QuadSys<NQ>::chi2(gptr(f.qcoef), f.quad_neq, x);
 }
 
 // The chain's nprm proposals of this iteration (one lane); returns the
@@ -480,41 +477,53 @@ __global__ void __launch_bounds__(256) k_eval_quad(const FitDesc* fits, const Wo
 template <int NQ>
 __device__ unsigned quad_one_chain(const FitDesc& f, int ch, int it, const TraceBuf& tr) {
   const int C = f.nchain, P = f.nprm;
+  const CONSTANT float* qc = wave_const(f.qcoef);
+  const int neq = __builtin_amdgcn_readfirstlane(f.quad_neq);
+  GLOBAL float* pcur = gptr(f.pcur);
+  GLOBAL float* pstp = gptr(f.pstp);
+  GLOBAL int* n_acpt = gptr(f.n_acpt);
+  GLOBAL int* n_cnt = gptr(f.n_cnt);
+  GLOBAL int* sumh = gptr(f.sumh);
+  GLOBAL uint32_t* hbits = gptr(f.hbits);
+  const CONSTANT int* ivar = wave_const(f.ivar);
+  const CONSTANT int* pdescr = wave_const(f.pdescr);
+  const CONSTANT float* pmint = wave_const(f.pmint);
+  const CONSTANT float* pmaxt = wave_const(f.pmaxt);
   const int ibeta = ch / f.nseq;
   const bool burn = it < f.nburn;
   float x[NQ];
 #pragma unroll
-  for (int i = 0; i < NQ; ++i) x[i] = f.pcur[(size_t)ch * NQ + i];
-  const double beta = (double)f.beta[ibeta];
-  float chi2c = f.chi2c[ch];
+  for (int i = 0; i < NQ; ++i) x[i] = pcur[(size_t)ch * NQ + i];
+  const double beta = (double)gptr(f.beta)[ibeta];
+  float chi2c = gptr(f.chi2c)[ch];
   int flag = 0, tlast = 0;
   float told = 0.0f;
   unsigned nev = 0;
   for (int s = 0; s < P; ++s) {
     const U4 r = philox4x32_10((uint32_t)ch, (uint32_t)it, (uint32_t)s, 0u, f.key0, f.key1);
     const int k = pick_with_retry(r.x, P, (uint32_t)ch, (uint32_t)it, (uint32_t)s, 0u, f.key0, f.key1);
-    const int t = f.ivar[k];
+    const int t = ivar[k];
     const size_t pk = (size_t)k * C + ch;
     float old = 0.0f;
 #pragma unroll
     for (int i = 0; i < NQ; ++i) old = (i == t) ? x[i] : old;
-    const float sd = f.pstp[pk];
+    const float sd = pstp[pk];
     double pd = (double)old + (double)sd * normal64(r.y, r.z);
-    if (f.pdescr[t] == 2) pd = pd - 6.283185307179586 * rint(pd / 6.283185307179586);
+    if (pdescr[t] == 2) pd = pd - 6.283185307179586 * rint(pd / 6.283185307179586);
     const float pn = (float)pd;
-    const bool inprior = !(pn < f.pmint[t] || pn > f.pmaxt[t]);
+    const bool inprior = !(pn < pmint[t] || pn > pmaxt[t]);
     int acc = 0;
     double c2 = 0.0;
     if (inprior) {
 #pragma unroll
       for (int i = 0; i < NQ; ++i) x[i] = (i == t) ? pn : x[i];
-      c2 = QuadSys<NQ>::chi2(f.qcoef, f.quad_neq, x);
+      c2 = QuadSys<NQ>::chi2(qc, neq, x);
       ++nev;
       const double xx = -0.5 * beta * (c2 - (double)chi2c);
       if (log((double)u01(r.w)) <= xx) {
         acc = 1;
         chi2c = (float)c2;
-        f.n_acpt[pk] += 1;
+        n_acpt[pk] += 1;
       } else {
 #pragma unroll
         for (int i = 0; i < NQ; ++i) x[i] = (i == t) ? old : x[i];
@@ -524,32 +533,32 @@ __device__ unsigned quad_one_chain(const FitDesc& f, int ch, int it, const Trace
     told = old;
     tlast = t;
     if (burn) {
-      const int cnt = f.n_cnt[pk];
+      const int cnt = n_cnt[pk];
       const int a = cnt % f.nadj;   // oldest history entry, replaced by this outcome
-      uint32_t* hw = f.hbits + ((size_t)(a >> 5) * P + k) * C + ch;
+      GLOBAL uint32_t* hw = hbits + ((size_t)(a >> 5) * P + k) * C + ch;
       const uint32_t word = *hw, bit = 1u << (a & 31);
       const int hold = (word & bit) ? 1 : 0;
       *hw = acc ? (word | bit) : (word & ~bit);
-      const int sh = f.sumh[pk] + acc - hold;
-      f.sumh[pk] = sh;
+      const int sh = sumh[pk] + acc - hold;
+      sumh[pk] = sh;
       const int den = cnt < f.nadj ? cnt : f.nadj;
       const float rate = (float)sh / (float)den;
-      const float st = f.pstp[pk];
-      f.pstp[pk] = rate < f.adapt_target ? (float)((double)st / 1.01) : (float)((double)st * 1.01);
-      f.n_cnt[pk] = cnt + 1;
+      const float st = pstp[pk];
+      pstp[pk] = rate < f.adapt_target ? (float)((double)st / 1.01) : (float)((double)st * 1.01);
+      n_cnt[pk] = cnt + 1;
     }
     if (tr.dec && it >= tr.it0 && it < tr.it0 + tr.nit) {
       const size_t ti = ((size_t)(it - tr.it0) * P + s) * C + ch;
-      tr.dec[ti] = k | ((inprior ? 1 : 0) << 8) | (acc << 9);
-      tr.chi2[ti] = inprior ? (float)c2 : 0.0f;
+      gptr(tr.dec)[ti] = k | ((inprior ? 1 : 0) << 8) | (acc << 9);
+      gptr(tr.chi2)[ti] = inprior ? (float)c2 : 0.0f;
     }
   }
 #pragma unroll
-  for (int i = 0; i < NQ; ++i) f.pcur[(size_t)ch * NQ + i] = x[i];
-  f.chi2c[ch] = chi2c;
-  f.flag[ch] = flag;
-  f.ptent[ch] = told;
-  f.ptentn[ch] = tlast;
+  for (int i = 0; i < NQ; ++i) pcur[(size_t)ch * NQ + i] = x[i];
+  gptr(f.chi2c)[ch] = chi2c;
+  gptr(f.flag)[ch] = flag;
+  gptr(f.ptent)[ch] = told;
+  gptr(f.ptentn)[ch] = tlast;
   return nev;
 }
 
@@ -564,7 +573,8 @@ __global__ void __launch_bounds__(256) k_step_quad(const FitDesc* fits, const Wo
   const WorkItem w = work[g];
   // every lane of a wave belongs to the same fit: its coefficients and sizes
   // are read through scalar loads
-  const FitDesc& f = fits[__builtin_amdgcn_readfirstlane(w.fit)];
+  const FitDesc& f = wave_fit(fits, w.fit);
+  pass_clock(clk);
   const int it = __builtin_amdgcn_readfirstlane(load_clock(clk));
   if (it >= f.nburn + f.niter) return;
   const bool live = w.idx >= 0;   // padding lanes only take part in the counting
@@ -679,29 +689,24 @@ __device__ void exchange_serial(const FitDesc& f, int iseq, int it, bool burn, c
 
 __global__ void __launch_bounds__(64) k_exchange(const FitDesc* fits, const WorkItem* xwork, XcdRanges xr,
                                                  RunClock* clk, TraceBuf tr) {
-  const int it = load_clock(clk);
+  const int it = __atomic_load_n(&clk->xit, __ATOMIC_RELAXED);
   const int xb = (int)(blockIdx.x % NXCD);
   const int wv = xr.off[xb] + (int)(blockIdx.x / NXCD);
   if (wv < xr.off[xb + 1]) {
     const WorkItem w = xwork[wv];
-    const FitDesc& f = fits[w.fit];
+    const FitDesc& f = wave_fit(fits, w.fit);
     if (it < f.nburn + f.niter) {
       const bool burn = it < f.nburn;
       if (f.nbeta <= 64) exchange_wave(f, w.idx, it, burn, tr);
       else exchange_serial(f, w.idx, it, burn, tr);
     }
   }
-  // Advance the iteration clock once every work-group of this launch has
-  // counted itself.  Only the next launches read the clock and the queue
-  // heads, and the kernel boundary orders them after these atomics, so no
-  // fence is needed here.
-  if (threadIdx.x == 0) {
-    const int v = atomicAdd(&clk->done, 1);
-    if (v == (int)gridDim.x - 1) {
-      atomicExch(&clk->done, 0);
-      for (int q = 0; q < NQSLOT * NXCD; ++q) atomicExch(&clk->qhead[q], 0);
-      atomicAdd(&clk->it, 1);
-    }
+  // The step launches of this iteration have finished and no work-group of
+  // this launch reads clk->it or the queue heads: one thread sets them up
+  // for the next iteration.
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    for (int q = 0; q < NQSLOT * NXCD; ++q) clk->qhead[q] = 0;
+    __atomic_store_n(&clk->it, it + 1, __ATOMIC_RELAXED);
   }
 }
 

@@ -95,11 +95,17 @@ struct XcdRanges {
 };
 
 // Iteration clock shared by the step and exchange kernels so that a cycle of
-// launches can be captured once in a hipGraph and replayed.
+// launches can be captured once in a hipGraph and replayed.  Each kind of
+// launch reads its own field and writes the other kind's, which no running
+// work-group reads (launches on one stream never overlap):
+//   step launch:     it = clk.it;  one thread stores clk.xit = it
+//   exchange launch: it = clk.xit; one thread stores clk.it = it + 1 and
+//                    clears the queue heads for the next step launches
+// so no launch has to count its finished work-groups.
 constexpr int NQSLOT = 6;
 struct RunClock {
-  int it;                     // global iteration index (burn-in first, then sampling)
-  int done;                   // exchange work-groups finished in the current iteration
+  int it;                     // iteration of the next step launches (burn-in first, then sampling)
+  int xit;                    // iteration of the next exchange launch
   int qhead[NQSLOT * NXCD];   // work-queue heads of the persistent step kernels, per XCD range
 };
 

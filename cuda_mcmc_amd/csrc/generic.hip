@@ -152,8 +152,9 @@ __global__ void __launch_bounds__(64) k_step_user(const FitDesc* fits, const Wor
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (__builtin_amdgcn_readfirstlane(g) >= nwork) return;
   const WorkItem w = work[g];
-  const FitDesc& f = fits[__builtin_amdgcn_readfirstlane(w.fit)];
-  const int it = __builtin_amdgcn_readfirstlane(__atomic_load_n(&clk->it, __ATOMIC_RELAXED));
+  const FitDesc& f = wave_fit(fits, w.fit);
+  pass_clock(clk);
+  const int it = __builtin_amdgcn_readfirstlane(load_clock(clk));
   if (it >= f.nburn + f.niter) return;
   const bool live = w.idx >= 0;
   const unsigned nev = live ? user_one_chain(f, w.idx, it, tr) : 0u;

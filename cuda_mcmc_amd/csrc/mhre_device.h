@@ -171,6 +171,34 @@ __device__ inline void load_bessel_tab(float4* tab) {
   __syncthreads();
 }
 
+// ------------------------------------------------------- address spaces
+
+// Global address space for the device pass (plain pointers in the host pass,
+// which never dereferences them): loads become global_load with scalar
+// bases instead of flat loads, which also count against the LDS counter.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define GLOBAL __attribute__((address_space(1)))
+#define CONSTANT __attribute__((address_space(4)))
+#else
+#define GLOBAL
+#define CONSTANT
+#endif
+template <class T>
+__device__ inline GLOBAL T* gptr(T* p) { return (GLOBAL T*)p; }
+
+// A pointer to data that no kernel writes while it runs, known by the caller
+// to be the same on every lane: made provably wave-uniform (scalar registers)
+// in the constant address space, so loads through it with uniform offsets
+// become scalar loads whose results feed the VALU from scalar registers.
+template <class T>
+__device__ inline const CONSTANT T* wave_const(const T* p) {
+  const uint64_t v = (uint64_t)p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+  return (const CONSTANT T*)(((uint64_t)hi << 32) | lo);
+}
+
+
 // ---------------------------------------------------------------- wave64
 
 __device__ inline int lane_id() { return (int)(threadIdx.x & 63u); }
