@@ -317,6 +317,15 @@ XcdRanges deal_xcd(std::vector<WorkItem>& items, const std::vector<double>& fit_
 
 // ------------------------------------------------- acceptance history bits
 
+// Lanes per sequence in the exchange kernel: ladders of up to 32 rungs take
+// the next power of two, so that 2 to 64 sequences share a wave.
+int exchange_lanes(int nbeta) {
+  if (nbeta > 32) return 64;
+  int w = 1;
+  while (w < nbeta) w <<= 1;
+  return w;
+}
+
 int hist_words(int nadj) { return (nadj + 31) / 32; }
 
 // n_hist[a][q] (0/1, q = k * nchain + chain) -> bit a % 32 of word [a / 32][q]
@@ -812,6 +821,7 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
     d.pcur = at<float>(S, h.o_pcur); d.chi2c = at<float>(S, h.o_chi2c); d.pstp = at<float>(S, h.o_pstp);
     d.n_cnt = at<int>(S, h.o_ncnt); d.n_acpt = at<int>(S, h.o_nacpt); d.hbits = at<uint32_t>(S, h.o_nhist);
     d.hwords = hist_words(s.nadj);
+    d.xlanes = exchange_lanes(s.nbeta);
     d.n_exch = at<int>(S, h.o_nexch); d.sumh = at<int>(S, h.o_sumh); d.tcur = at<int>(S, h.o_tcur);
     d.flag = at<int>(S, h.o_flag); d.ptent = at<float>(S, h.o_ptent); d.ptentn = at<int>(S, h.o_ptentn);
     d.pout = at<float>(S, h.o_pout); d.tout = at<int>(S, h.o_tout); d.chi2 = at<float>(S, h.o_chi2);
@@ -917,7 +927,7 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
   std::vector<double> seq_cost(nfits);
   for (int i = 0; i < nfits; ++i) {
     const FitHost& h = S->fits[i];
-    seq_cost[i] = (double)h.s.nbeta;
+    seq_cost[i] = (double)h.s.nbeta * (64 / exchange_lanes(h.s.nbeta));   // per wave
     if (h.s.init_chains)
       for (int c = 0; c < h.nchain; ++c) init.push_back({i, c});
   }
@@ -926,8 +936,11 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
     std::vector<int> order(nfits);
     for (int i = 0; i < nfits; ++i) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return seq_cost[a] > seq_cost[b]; });
-    for (int i : order)
-      for (int q = 0; q < S->fits[i].s.nseq; ++q) xw.push_back({i, q});
+    // one item per wave: its first sequence (64 / xlanes sequences per wave)
+    for (int i : order) {
+      const int g = 64 / exchange_lanes(S->fits[i].s.nbeta);
+      for (int q = 0; q < S->fits[i].s.nseq; q += g) xw.push_back({i, q});
+    }
     S->xr_x = deal_xcd(xw, seq_cost);
   }
   for (auto& k : S->kinds) {

@@ -25,9 +25,20 @@ struct RingInv {
   float se, rq2;           // ye = se * rho^2, yi = rq2 * ye
   float kBe, kCe, kBi, kCi;
   float kd;                // inner-disk shift phase per unit U: 2 pi d
+  float kdp;               // the same in units of pi: 2 d
   float cr, cg;            // ring and Gaussian mixing weights (Zsp folded in)
   int gauss;               // Gaussian present
+  float mBe, mCe, mBi, mCi;   // cr * (kBe, kCe, kBi, kCi): the ring term already mixed
 };
+
+// The mixed ring coefficients, once cr and the k's are known (rounded to
+// fp32 from the fp32 factors, as the per-visibility product would be).
+__device__ inline void mix_ring(RingInv* q) {
+  q->mBe = q->cr * q->kBe;
+  q->mCe = q->cr * q->kCe;
+  q->mBi = q->cr * q->kBi;
+  q->mCi = q->cr * q->kCi;
+}
 
 // h, h0 and the ring coefficients (model.cuh:366-375) for radii in nrad.
 __device__ inline void ring_coefs(double Rex, double rq, double fade, double d, RingInv* q) {
@@ -42,9 +53,10 @@ __device__ inline void ring_coefs(double Rex, double rq, double fade, double d, 
   q->kBi = (float)(D_PI * (h + h0 - d * (h - h0) / Rex) * Rin * Rin);
   q->kCi = (float)(-2.0 * D_PI * D_PI * (h - h0) * rq * Rin * Rin * Rin);
   q->kd = (float)(2.0 * D_PI * d);
+  q->kdp = (float)(2.0 * d);
 }
 
-// Ring visibility Fr = Fex - Fin at rotated coordinates (U, V).
+// Ring visibility cr (Fex - Fin) at rotated coordinates (U, V).
 // any_large == false: the caller guarantees ye <= 64 on every lane, so only
 // the y <= 64 evaluation bc of B, C runs (BesselPoly or BesselTab).
 template <class BC>
@@ -62,15 +74,15 @@ __device__ inline float2 ring_vis(const RingInv& q, float U, float V, bool any_l
     if (yi <= 64.0f) bc(yi, &Bi, &Ci);
     else bessel_BC_asym(yi, &Bi, &Ci);
   }
-  const float ree = q.kBe * Be;
-  const float ime = q.kCe * U * Ce;
-  const float rei = q.kBi * Bi;
-  const float imi = q.kCi * U * Ci;
+  const float ree = q.mBe * Be;
+  const float ime = q.mCe * U * Ce;
+  const float rei = q.mBi * Bi;
+  const float imi = q.mCi * U * Ci;
   float sd, cd;
-  sincos_fast(q.kd * U, &sd, &cd);
+  sincospi_fast(q.kdp * U, &sd, &cd);
   float2 F;
-  F.x = ree - (rei * cd - imi * sd);
-  F.y = ime - (rei * sd + imi * cd);
+  F.x = fmaf(imi, sd, fmaf(-rei, cd, ree));
+  F.y = fmaf(-imi, cd, fmaf(-rei, sd, ime));
   return F;
 }
 
@@ -82,6 +94,7 @@ struct Sgra9 {
     float cth, sth;       // rotation by th
     float gaU, gaV;       // Gaussian exponent (log2 units) per U^2, V^2
     float kg;             // Gaussian shift phase per unit U: -2 pi (Ri - d)
+    float kgp;            // the same in units of pi: -2 (Ri - d)
   };
 
   // p[i] = parameter i (Zsp, Re, rq, ecc, fade, gax, aq, gq, th).
@@ -93,6 +106,7 @@ struct Sgra9 {
     const double d = ecc * (Rex - Rin);
     ring_coefs(Rex, rq, fade, d, &q.r);
     q.kg = (float)(-2.0 * D_PI * (Rin - d));
+    q.kgp = (float)(-2.0 * (Rin - d));
   }
   __device__ static void rotation(Inv& q, const float* p) {   // th
     q.cth = (float)cos((double)p[8]);
@@ -110,6 +124,7 @@ struct Sgra9 {
     q.r.gauss = nog ? 0 : 1;
     q.r.cr = (float)(nog ? Zsp : Zsp * (1.0 - gq));
     q.r.cg = (float)(nog ? 0.0 : Zsp * gq);
+    mix_ring(&q.r);
   }
 
   __device__ static Inv prepare(const float* p) {
@@ -125,18 +140,14 @@ struct Sgra9 {
   __device__ static float2 vis(const Inv& q, float u, float v, bool any_large, const BC& bc = BC()) {
     const float U = fmaf(u, q.cth, v * q.sth);
     const float V = fmaf(v, q.cth, -u * q.sth);
-    const float2 F = ring_vis(q.r, U, V, any_large, bc);
-    float2 out;
+    float2 out = ring_vis(q.r, U, V, any_large, bc);
     if (q.r.gauss) {
       const float ga = __builtin_amdgcn_exp2f(-fmaf(q.gaU, U * U, q.gaV * (V * V)));
       float sg, cgs;
-      sincos_fast(q.kg * U, &sg, &cgs);
+      sincospi_fast(q.kgp * U, &sg, &cgs);
       const float gw = q.r.cg * ga;
-      out.x = fmaf(q.r.cr, F.x, gw * cgs);
-      out.y = fmaf(q.r.cr, F.y, gw * sg);
-    } else {
-      out.x = q.r.cr * F.x;
-      out.y = q.r.cr * F.y;
+      out.x = fmaf(gw, cgs, out.x);
+      out.y = fmaf(gw, sg, out.y);
     }
     return out;
   }
@@ -150,7 +161,7 @@ struct Sgra13 {
     float cth, sth;       // ring rotation by th
     float cb, sb;         // Gaussian rotation by beta
     float gaU, gaV;
-    float kgu, kgv;       // Gaussian phase per unit (u, v): -2 pi gr Re (cos alpha, sin alpha)
+    float kgu, kgv;       // Gaussian phase per unit (u, v) in units of pi: -2 gr Re (cos alpha, sin alpha)
     int gonly;            // gq == 1: Gaussian only
   };
 
@@ -162,16 +173,14 @@ struct Sgra13 {
     if (!q.gonly) {
       const float U = fmaf(u, q.cth, v * q.sth);
       const float V = fmaf(v, q.cth, -u * q.sth);
-      const float2 F = ring_vis(q.r, U, V, any_large, bc);
-      out.x = q.r.cr * F.x;
-      out.y = q.r.cr * F.y;
+      out = ring_vis(q.r, U, V, any_large, bc);
     }
     if (q.r.gauss) {
       const float Ub = fmaf(u, q.cb, v * q.sb);
       const float Vb = fmaf(v, q.cb, -u * q.sb);
       const float ga = __builtin_amdgcn_exp2f(-fmaf(q.gaU, Ub * Ub, q.gaV * (Vb * Vb)));
       float sg, cgs;
-      sincos_fast(fmaf(q.kgu, u, q.kgv * v), &sg, &cgs);
+      sincospi_fast(fmaf(q.kgu, u, q.kgv * v), &sg, &cgs);
       const float gw = q.r.cg * ga;
       out.x = fmaf(gw, cgs, out.x);
       out.y = fmaf(gw, sg, out.y);
@@ -198,13 +207,14 @@ __device__ inline Sgra13::Inv Sgra13::prepare(const float* p) {
   const double gsy = gax * Rex, gsx = aq * gsy;
   q.gaU = (float)(GCOEF * LOG2E * gsx * gsx);
   q.gaV = (float)(GCOEF * LOG2E * gsy * gsy);
-  q.kgu = (float)(-2.0 * D_PI * gr * Rex * cos(alpha));
-  q.kgv = (float)(-2.0 * D_PI * gr * Rex * sin(alpha));
+  q.kgu = (float)(-2.0 * gr * Rex * cos(alpha));
+  q.kgv = (float)(-2.0 * gr * Rex * sin(alpha));
   q.gonly = (p[8] == 1.0f) ? 1 : 0;
   q.r.gauss = gauss ? 1 : 0;
   q.r.cr = (float)(gauss ? Zsp * (1.0 - gq) : Zsp);
   q.r.cg = (float)(gauss ? Zsp * gq : 0.0);
   if (q.gonly) { q.r.cr = 0.0f; q.r.cg = (float)(gauss ? Zsp : 0.0); }
+  mix_ring(&q.r);
   return q;
 }
 

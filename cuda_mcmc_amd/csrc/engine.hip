@@ -584,38 +584,47 @@ __global__ void __launch_bounds__(256) k_step_quad(const FitDesc* fits, const Wo
 
 // ---------------------------------------------------------------- exchange
 
-__device__ void exchange_wave(const FitDesc& f, int iseq, int it, bool burn, const TraceBuf& tr) {
+// Sequences iseq0 .. iseq0 + 64/W - 1 of the fit, W = f.xlanes lanes each:
+// lane g*W + j holds temperature slot j of sequence iseq0 + g.  The swap
+// attempts of one sequence are sequential (reference: gpu_mcmc.cu:548-624);
+// the groups of a wave run theirs side by side, exchanging values with lane
+// shuffles inside each group.
+__device__ void exchange_wave(const FitDesc& f, int iseq0, int it, bool burn, const TraceBuf& tr) {
   const int lane = lane_id();
-  const int B = f.nbeta, S = f.nseq, N = f.nptot;
-  const bool slot = lane < B;
-  const size_t me = (size_t)lane * S + iseq;
+  const int B = f.nbeta, S = f.nseq, N = f.nptot, W = f.xlanes;
+  const int sl = lane & (W - 1);   // temperature slot
+  const int gb = lane - sl;        // first lane of this sequence's group
+  const int iseq = iseq0 + lane / W;
+  const bool seq_ok = iseq < S;
+  const bool slot = seq_ok && sl < B;
+  const size_t me = (size_t)sl * S + iseq;
   float c2 = slot ? f.chi2c[me] : 0.0f;
   int tc = slot ? f.tcur[me] : 0;
-  int perm = lane;
+  int perm = sl;
   int nex = 0;
   if (B > 1) {
-    // attempt a's pair pick and log u on lane a, all drawn up front
+    // attempt a's pair pick and log u on slot a, all drawn up front
     int k_l = 0;
     double lu_l = 0.0, hb_l = 0.0;
     if (slot) {
-      const U4 r = philox4x32_10((uint32_t)iseq, (uint32_t)it, (uint32_t)lane, RNG_EXCH, f.key0, f.key1);
-      k_l = pick_with_retry(r.x, B - 1, (uint32_t)iseq, (uint32_t)it, (uint32_t)lane, RNG_EXCH, f.key0, f.key1);
+      const U4 r = philox4x32_10((uint32_t)iseq, (uint32_t)it, (uint32_t)sl, RNG_EXCH, f.key0, f.key1);
+      k_l = pick_with_retry(r.x, B - 1, (uint32_t)iseq, (uint32_t)it, (uint32_t)sl, RNG_EXCH, f.key0, f.key1);
       lu_l = log((double)u01(r.y));
     }
-    // -0.5 (beta[k+1] - beta[k]) on lane k
-    if (lane < B - 1) hb_l = -0.5 * ((double)f.beta[lane + 1] - (double)f.beta[lane]);
-    for (int a = 0; a < B; ++a) {
-      const int k = bcast(k_l, a);
-      const float ck = bcast(c2, k), ck1 = bcast(c2, k + 1);
-      const double x = bcast(hb_l, k) * ((double)ck - (double)ck1);
-      const bool acc = bcast(lu_l, a) <= x;
+    // -0.5 (beta[k+1] - beta[k]) on slot k
+    if (sl < B - 1) hb_l = -0.5 * ((double)f.beta[sl + 1] - (double)f.beta[sl]);
+    for (int a = 0; a < B; ++a) {   // every lane takes part in every shuffle
+      const int k = __shfl(k_l, gb + a);
+      const float ck = __shfl(c2, gb + k), ck1 = __shfl(c2, gb + k + 1);
+      const double x = __shfl(hb_l, gb + k) * ((double)ck - (double)ck1);
+      const bool acc = __shfl(lu_l, gb + a) <= x;
+      const int tk = __shfl(tc, gb + k), tk1 = __shfl(tc, gb + k + 1);
+      const int pk = __shfl(perm, gb + k), pk1 = __shfl(perm, gb + k + 1);
       if (acc) {
-        const int tk = bcast(tc, k), tk1 = bcast(tc, k + 1);
-        const int pk = bcast(perm, k), pk1 = bcast(perm, k + 1);
-        if (lane == k) { c2 = ck1; tc = tk1; perm = pk1; if (!burn) ++nex; }
-        if (lane == k + 1) { c2 = ck; tc = tk; perm = pk; }
+        if (sl == k) { c2 = ck1; tc = tk1; perm = pk1; if (!burn) ++nex; }
+        if (sl == k + 1) { c2 = ck; tc = tk; perm = pk; }
       }
-      if (tr.ex && lane == 0 && it >= tr.it0 && it < tr.it0 + tr.nit)
+      if (tr.ex && sl == 0 && seq_ok && it >= tr.it0 && it < tr.it0 + tr.nit)
         tr.ex[((size_t)(it - tr.it0) * B + a) * S + iseq] = k | ((acc ? 1 : 0) << 16);
     }
   }
@@ -624,7 +633,7 @@ __device__ void exchange_wave(const FitDesc& f, int iseq, int it, bool burn, con
 #pragma unroll
   for (int j = 0; j < MAXP; ++j)
     row[j] = (slot && j < N) ? f.pcur[((size_t)perm * S + iseq) * N + j] : 0.0f;
-  if (slot && perm != lane) {
+  if (slot && perm != sl) {
 #pragma unroll
     for (int j = 0; j < MAXP; ++j)
       if (j < N) f.pcur[me * N + j] = row[j];
@@ -641,12 +650,12 @@ __device__ void exchange_wave(const FitDesc& f, int iseq, int it, bool burn, con
       for (int j = 0; j < MAXP; ++j) {
         if (j < N) {
           const int ip = f.invar[j];
-          if (ip >= 0) f.pout[(((size_t)B * ip + lane) * S + iseq) * f.niter + itr] = row[j];
+          if (ip >= 0) f.pout[(((size_t)B * ip + sl) * S + iseq) * f.niter + itr] = row[j];
         }
       }
       f.tout[me * f.niter + itr] = tc;
     }
-    if (lane == 0) f.chi2[(size_t)iseq * f.niter + itr] = c2;
+    if (sl == 0 && seq_ok) f.chi2[(size_t)iseq * f.niter + itr] = c2;
   }
 }
 

@@ -48,6 +48,8 @@ struct FitDesc {
   int quad_n, quad_neq;  // MODEL_QUADSYS sizes
   int record;            // 1: write pout/tout/chi2 in the sampling stage
   int hwords;            // words per acceptance history: ceil(nadj / 32)
+  int xlanes;            // exchange: lanes per sequence (a power of two >= nbeta, or 64),
+                         // so one wave swaps 64 / xlanes sequences side by side
   long long ndatm;       // length of datm (head nchain*ndat, then the phase tail)
 
   // ---- static inputs (device pointers)

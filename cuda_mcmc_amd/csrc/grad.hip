@@ -97,7 +97,9 @@ struct Fwd9 {
 };
 
 // Forward model of one visibility, keeping what the backward pass needs.
-// Same arithmetic as Sgra9::vis with the full-range Bessel branch.
+// The model of Sgra9::vis with the full-range Bessel branch; phases go
+// through sincos_fast in radians (Sgra9::vis uses sincospi_fast on the same
+// phases in units of pi), and the ring term is mixed after the fact.
 __device__ inline Fwd9 sgra9_fwd(const Sgra9::Inv& q, float u, float v) {
   Fwd9 f;
   f.u = u;

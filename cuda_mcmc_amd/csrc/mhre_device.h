@@ -101,6 +101,21 @@ __device__ inline void sincos_fast(float theta, float* s, float* c) {
   *c = __uint_as_float(__float_as_uint(cs) ^ flip);
 }
 
+// sin(pi t) and cos(pi t): k = rint(t), r = t - k in [-1/2, 1/2] (exact),
+// least-squares kernels in r of degree 9 / 8, sign (-1)^k.  Callers scale
+// their phase constants by 1/pi once per chain, which saves the reduction's
+// multiply and second Cody-Waite step of sincos_fast per evaluation.
+__device__ inline void sincospi_fast(float t, float* s, float* c) {
+  const float k = rintf(t);
+  const float r = t - k;
+  const float r2 = r * r;
+  const float sn = r * horner(SINPI_T, r2);
+  const float cs = fmaf(r2, horner(COSPI_T, r2), 1.0f);
+  const uint32_t flip = ((uint32_t)(int)k & 1u) << 31;
+  *s = __uint_as_float(__float_as_uint(sn) ^ flip);
+  *c = __uint_as_float(__float_as_uint(cs) ^ flip);
+}
+
 // atan2(y, x) in (-pi, pi]: t = min/max in [0,1], atan(t) = t P(t^2), octant fix-up.
 __device__ inline float atan2_fast(float y, float x) {
   const float ax = fabsf(x), ay = fabsf(y);
@@ -156,7 +171,7 @@ struct BesselTab {
   const float4* tab;
   __device__ void operator()(float y, float* B, float* C) const {
     const int j = (int)y;
-    const float s = y - (float)j;
+    const float s = __builtin_amdgcn_fractf(y);   // y - j exactly, y >= 0
     const float4 b = tab[2 * j], c = tab[2 * j + 1];
     *B = fmaf(fmaf(fmaf(b.w, s, b.z), s, b.y), s, b.x);
     *C = fmaf(fmaf(fmaf(c.w, s, c.z), s, c.y), s, c.x);
