@@ -59,20 +59,23 @@ __device__ inline void ring_coefs(double Rex, double rq, double fade, double d, 
 // Ring visibility cr (Fex - Fin) at rotated coordinates (U, V).
 // any_large == false: the caller guarantees ye <= 64 on every lane, so only
 // the y <= 64 evaluation bc of B, C runs (BesselPoly or BesselTab).
+// bc takes its argument in units of 1/BC::SCALE (the scale is folded into se
+// once per chain, outside the visibility loop).
 template <class BC>
 __device__ inline float2 ring_vis(const RingInv& q, float U, float V, bool any_large, const BC& bc) {
+  constexpr float S = BC::SCALE;
   const float rho2 = fmaf(U, U, V * V);
-  const float ye = q.se * rho2;
+  const float ye = (q.se * S) * rho2;
   const float yi = q.rq2 * ye;
   float Be, Ce, Bi, Ci;
   if (!any_large) {
     bc(ye, &Be, &Ce);
     bc(yi, &Bi, &Ci);
   } else {
-    if (ye <= 64.0f) bc(ye, &Be, &Ce);
-    else bessel_BC_asym(ye, &Be, &Ce);
-    if (yi <= 64.0f) bc(yi, &Bi, &Ci);
-    else bessel_BC_asym(yi, &Bi, &Ci);
+    if (ye <= 64.0f * S) bc(ye, &Be, &Ce);
+    else bessel_BC_asym(ye * (1.0f / S), &Be, &Ce);
+    if (yi <= 64.0f * S) bc(yi, &Bi, &Ci);
+    else bessel_BC_asym(yi * (1.0f / S), &Bi, &Ci);
   }
   const float ree = q.mBe * Be;
   const float ime = q.mCe * U * Ce;

@@ -141,14 +141,13 @@ __device__ double eht_chi2_wave(const EhtView& v, const typename M::Inv& q, floa
   // The packed arrays are padded (vis by one tile of zero weight, closures by
   // one wave, tinfo by two entries), so every load below is in bounds and the
   // loads can stay in flight across the model evaluation.
-  float4 dn = ldg(v.vis, lane);
   int c0 = 0;           // first closure not yet formed
   int kleft = v.cgroup;
   int2 ti = ldg(v.tinfo, 1);
   float r2cur = __int_as_float(ldg(v.tinfo, 0).y);
-  for (int t = 0; t < v.ntile; ++t) {
+  // tile t with its data d; fetches tile t + 1 into dn
+  auto tile = [&](const int t, const float4& d, float4& dn) {
     const int i = t * TILE + lane;
-    const float4 d = dn;
     dn = ldg(v.vis, i + TILE);
     const int2 tin = ldg(v.tinfo, t + 2);
     const int cf = c0 + lane;
@@ -197,6 +196,13 @@ __device__ double eht_chi2_wave(const EhtView& v, const typename M::Inv& q, floa
     }
     r2cur = __int_as_float(ti.y);
     ti = tin;
+  };
+  // two tiles per trip: the prefetched tile alternates between da and db,
+  // so the prefetch costs no register copies
+  float4 da = ldg(v.vis, lane), db;
+  for (int t = 0; t < v.ntile; t += 2) {
+    tile(t, da, db);
+    if (t + 1 < v.ntile) tile(t + 1, db, da);
   }
   if (v.ncph && !v.ring_ok) {
     // General closure layout: recompute the three model visibilities.

@@ -200,16 +200,18 @@ def main():
     mD = cheb_monomial(D, 0.0, 64.0, 8)
     eD = check32(mD, D, 0.0, 64.0)
     emit("BESSEL_D", mD, "J3(x)/x^3 = sum c_k t^k, t = x^2/32 - 1, x in [0,8]; fp32 max abs err %.2e" % eD)
-    tb = piecewise_cubic(B, 65)
-    tc = piecewise_cubic(C, 65)
-    eb = check_piecewise32(tb, B)
-    ec = check_piecewise32(tc, C)
-    tab = np.concatenate([tb, tc], axis=1).ravel()
-    print("// B and C on y in [j, j+1), j = 0..64: c0 + s(c1 + s(c2 + s c3)), s = y - j;")
-    print("// 8 floats per interval (B then C); fp32 max abs err B %.2e, C %.2e on [0, 64]" % (eb, ec))
-    print("static constexpr int BESSEL_TAB_N = 65;")
+    nq = 257   # quarter-unit intervals of y covering [0, 64] (the last one only at y = 64)
+    tb = piecewise_cubic(B, nq, h=0.25, deg=2)
+    tc = piecewise_cubic(C, nq, h=0.25, deg=2)
+    eb = check_piecewise32(tb, B, h=0.25)
+    ec = check_piecewise32(tc, C, h=0.25)
+    tab = np.concatenate([tb, tc, np.zeros((nq, 2))], axis=1).ravel()
+    print("// B and C on y in [j/4, (j+1)/4), j = 0..256: c0 + s(c1 + s c2), s = 4y - j;")
+    print("// 8 floats per interval (B0 B1 B2 C0 C1 C2 0 0); fp32 max abs err B %.2e, C %.2e on [0, 64]"
+          % (eb, ec))
+    print("static constexpr int BESSEL_TAB_N = %d;" % nq)
     print("__device__ static constexpr float BESSEL_TAB[%d] = {" % tab.size)
-    for j in range(65):
+    for j in range(nq):
         print("    " + " ".join("%.9ef," % v for v in tab[8 * j:8 * j + 8]))
     print("};")
     print()

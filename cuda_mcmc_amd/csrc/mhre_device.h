@@ -159,22 +159,26 @@ __device__ inline void bessel_BC(float y, float* B, float* C) {
   else bessel_BC_asym(y, B, C);
 }
 
-// B and C for y <= 64 from polynomials (no table needed).
+// B and C for y <= 64 from polynomials (no table needed).  The argument is y
+// itself (SCALE 1).
 struct BesselPoly {
+  static constexpr float SCALE = 1.0f;
   __device__ void operator()(float y, float* B, float* C) const { bessel_BC_poly(y, B, C); }
 };
 
-// B and C for y in [0, 64] from the piecewise cubics of BESSEL_TAB, staged in
-// LDS by the kernel (tab = 2 float4 per unit interval of y: B, then C
-// coefficients).  Interval j = trunc(y), s = y - j.
+// B and C for y in [0, 64] from the piecewise quadratics of BESSEL_TAB,
+// staged in LDS by the kernel (2 float4 per quarter-unit interval of y:
+// B0 B1 B2 C0, C1 C2 - -).  The argument is z = SCALE * y = 4y (callers fold
+// the 4 into their per-chain constants); interval j = trunc(z), s = z - j.
 struct BesselTab {
+  static constexpr float SCALE = 4.0f;
   const float4* tab;
-  __device__ void operator()(float y, float* B, float* C) const {
-    const int j = (int)y;
-    const float s = __builtin_amdgcn_fractf(y);   // y - j exactly, y >= 0
-    const float4 b = tab[2 * j], c = tab[2 * j + 1];
-    *B = fmaf(fmaf(fmaf(b.w, s, b.z), s, b.y), s, b.x);
-    *C = fmaf(fmaf(fmaf(c.w, s, c.z), s, c.y), s, c.x);
+  __device__ void operator()(float z, float* B, float* C) const {
+    const int j = (int)z;
+    const float s = __builtin_amdgcn_fractf(z);   // z - j exactly, z >= 0
+    const float4 a = tab[2 * j], b = tab[2 * j + 1];
+    *B = fmaf(fmaf(a.z, s, a.y), s, a.x);
+    *C = fmaf(fmaf(b.y, s, b.x), s, a.w);
   }
 };
 

@@ -75,14 +75,16 @@ def test_sinpi_cospi_kernels():
     assert np.max(np.abs(c - np.cos(np.pi * td))) < 2.5e-7
 
 
-def test_bessel_table_piecewise_cubics():
+def test_bessel_table_piecewise_quadratics():
     a = _arrays()
-    tab = a["BESSEL_TAB"].reshape(65, 8)
-    y = np.linspace(0.0, 64.0, 64001)[:-1]
-    j = y.astype(np.int64)
-    s = (y - j).astype(np.float32)
-    B = _horner32_rows(tab[j, 0:4], s)
-    C = _horner32_rows(tab[j, 4:8], s)
+    tab = a["BESSEL_TAB"].reshape(-1, 8)
+    assert tab.shape[0] == 257   # quarter-unit intervals of y over [0, 64]
+    z = np.linspace(0.0, 256.0, 256001)   # z = 4y, the kernel's argument
+    j = z.astype(np.int64)
+    s = (z - j).astype(np.float32)
+    B = _horner32_rows(tab[j, 0:3], s)
+    C = _horner32_rows(tab[j, 3:6], s)
+    y = z / 4.0
     x = np.sqrt(y)
     xs = np.maximum(x, 1e-3)
     Bt = np.where(x > 1e-3, j1(xs) / xs, 0.5 - y / 16.0)
@@ -90,12 +92,12 @@ def test_bessel_table_piecewise_cubics():
     assert np.max(np.abs(B - Bt)) < 6e-8
     assert np.max(np.abs(C - Ct)) < 2e-8
     # B(0) = 1/2 and C(0) = 1/8: the rho -> 0 limits of the reference model
-    assert tab[0, 0] == pytest.approx(0.5, abs=1e-7) and tab[0, 4] == pytest.approx(0.125, abs=1e-7)
+    assert tab[0, 0] == pytest.approx(0.5, abs=1e-7) and tab[0, 3] == pytest.approx(0.125, abs=1e-7)
 
 
 def _horner32_rows(c, s):
-    acc = c[:, 3].astype(np.float32)
-    for k in (2, 1, 0):
+    acc = c[:, -1].astype(np.float32)
+    for k in range(c.shape[1] - 2, -1, -1):
         acc = (acc * s + c[:, k].astype(np.float32)).astype(np.float32)
     return acc
 
