@@ -111,9 +111,15 @@ struct Sgra9 {
     q.kg = (float)(-2.0 * D_PI * (Rin - d));
     q.kgp = (float)(-2.0 * (Rin - d));
   }
-  __device__ static void rotation(Inv& q, const float* p) {   // th
-    q.cth = (float)cos((double)p[8]);
-    q.sth = (float)sin((double)p[8]);
+  // The float64 trigonometric invariants (cos th, sin th).  The step kernel
+  // keeps them from one proposal to the next and recomputes them only when
+  // the proposal moves a parameter they depend on (trig_uses).
+  struct Trig {
+    float cth, sth;
+  };
+  __device__ static bool trig_uses(int t) { return t == 8; }
+  __device__ static Trig trig(const float* p) {
+    return Trig{(float)cos((double)p[8]), (float)sin((double)p[8])};
   }
   __device__ static void gaussian(Inv& q, const float* p) {   // gax, aq (and Re)
     const double Rex = (double)p[1] * UAS2NRAD;
@@ -130,14 +136,16 @@ struct Sgra9 {
     mix_ring(&q.r);
   }
 
-  __device__ static Inv prepare(const float* p) {
+  __device__ static Inv prepare(const float* p, const Trig& tg) {
     Inv q;
     ring(q, p);
-    rotation(q, p);
+    q.cth = tg.cth;
+    q.sth = tg.sth;
     gaussian(q, p);
     mixing(q, p);
     return q;
   }
+  __device__ static Inv prepare(const float* p) { return prepare(p, trig(p)); }
 
   template <class BC = BesselPoly>
   __device__ static float2 vis(const Inv& q, float u, float v, bool any_large, const BC& bc = BC()) {
@@ -168,7 +176,17 @@ struct Sgra13 {
     int gonly;            // gq == 1: Gaussian only
   };
 
-  __device__ static Inv prepare(const float* p);
+  // float64 trigonometric invariants of alpha, beta, eta - th and th
+  // (parameters 9..12), kept by the step kernel across proposals that move
+  // none of them
+  struct Trig {
+    double ceth, ca, sa;
+    float cth, sth, cb, sb;
+  };
+  __device__ static bool trig_uses(int t) { return t >= 9; }
+  __device__ static Trig trig(const float* p);
+  __device__ static Inv prepare(const float* p, const Trig& tg);
+  __device__ static Inv prepare(const float* p) { return prepare(p, trig(p)); }
 
   template <class BC = BesselPoly>
   __device__ static float2 vis(const Inv& q, float u, float v, bool any_large, const BC& bc = BC()) {
@@ -193,25 +211,37 @@ struct Sgra13 {
 };
 
 // p: Zsp, Re, rq, ecc, fade, gr, gax, aq, gq, alpha, beta, eta, th
-__device__ inline Sgra13::Inv Sgra13::prepare(const float* p) {
+__device__ inline Sgra13::Trig Sgra13::trig(const float* p) {
+  const double alpha = p[9], beta = p[10], eta = p[11], th = p[12];
+  Trig tg;
+  tg.ceth = cos(eta - th);
+  tg.ca = cos(alpha);
+  tg.sa = sin(alpha);
+  tg.cth = (float)cos(th);
+  tg.sth = (float)sin(th);
+  tg.cb = (float)cos(beta);
+  tg.sb = (float)sin(beta);
+  return tg;
+}
+
+__device__ inline Sgra13::Inv Sgra13::prepare(const float* p, const Trig& tg) {
   Inv q;
   const double Zsp = p[0], Re = p[1], rq = p[2], ecc = p[3], fade = p[4];
   const double gr = p[5], gax = p[6], aq = p[7], gq = p[8];
-  const double alpha = p[9], beta = p[10], eta = p[11], th = p[12];
   const double Rex = Re * UAS2NRAD;
   const double Rin = rq * Rex;
-  const double d = ecc * (Rex - Rin) * cos(eta - th);
+  const double d = ecc * (Rex - Rin) * tg.ceth;
   ring_coefs(Rex, rq, fade, d, &q.r);
-  q.cth = (float)cos(th);
-  q.sth = (float)sin(th);
-  q.cb = (float)cos(beta);
-  q.sb = (float)sin(beta);
+  q.cth = tg.cth;
+  q.sth = tg.sth;
+  q.cb = tg.cb;
+  q.sb = tg.sb;
   const bool gauss = (p[6] != 0.0f) && (p[7] != 0.0f) && (p[8] != 0.0f);
   const double gsy = gax * Rex, gsx = aq * gsy;
   q.gaU = (float)(GCOEF * LOG2E * gsx * gsx);
   q.gaV = (float)(GCOEF * LOG2E * gsy * gsy);
-  q.kgu = (float)(-2.0 * gr * Rex * cos(alpha));
-  q.kgv = (float)(-2.0 * gr * Rex * sin(alpha));
+  q.kgu = (float)(-2.0 * gr * Rex * tg.ca);
+  q.kgv = (float)(-2.0 * gr * Rex * tg.sa);
   q.gonly = (p[8] == 1.0f) ? 1 : 0;
   q.r.gauss = gauss ? 1 : 0;
   q.r.cr = (float)(gauss ? Zsp * (1.0 - gq) : Zsp);

@@ -227,10 +227,14 @@ __device__ double eht_chi2_wave(const EhtView& v, const typename M::Inv& q, floa
 // Gather the chain's parameters (lane j holds parameter j) into a uniform
 // array and build the model invariants.
 template <class M>
-__device__ inline typename M::Inv prepare_from_lanes(float pl) {
-  float p[M::NP];
+__device__ inline void gather_params(float pl, float (&p)[M::NP]) {
 #pragma unroll
   for (int j = 0; j < M::NP; ++j) p[j] = bcast(pl, j);
+}
+template <class M>
+__device__ inline typename M::Inv prepare_from_lanes(float pl) {
+  float p[M::NP];
+  gather_params<M>(pl, p);
   return M::prepare(p);
 }
 
@@ -298,6 +302,13 @@ __device__ void step_one_chain(const FitDesc& f, int ch, int it, float2* ring, c
   float told = 0.0f;
   unsigned nev = 0;
   const EhtView view = eht_view(f);
+  // the float64 trigonometric invariants of the current state
+  typename M::Trig tg;
+  {
+    float p[M::NP];
+    gather_params<M>(pl, p);
+    tg = M::trig(p);
+  }
 
   // This iteration's random numbers, substep s on lane s: the coordinate
   // pick, the Gaussian step and log u of the accept test (drawn for every
@@ -324,13 +335,17 @@ __device__ void step_one_chain(const FitDesc& f, int ch, int it, float2* ring, c
     double c2 = 0.0;
     if (inprior) {
       if (lane == t) pl = pn;
-      const typename M::Inv q = prepare_from_lanes<M>(pl);
+      float p[M::NP];
+      gather_params<M>(pl, p);
+      const typename M::Trig tn = M::trig_uses(t) ? M::trig(p) : tg;
+      const typename M::Inv q = M::prepare(p, tn);
       c2 = eht_chi2_wave<M, false>(view, q, ring, bt, ch);
       ++nev;
       const double x = -0.5 * beta * (c2 - (double)chi2c);
       if (bcast(lu_l, s) <= x) {
         acc = 1;
         chi2c = (float)c2;
+        tg = tn;
         if (lane == k) ++nac;
       } else if (lane == t) {
         pl = old;

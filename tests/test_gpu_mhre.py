@@ -136,11 +136,12 @@ def test_lockstep_adaptation_threshold_and_short_history():
     assert compared >= 9 * 16 * 0.5
 
 
-@pytest.mark.parametrize("nbeta,nseq", [(3, 7), (5, 13), (40, 3), (70, 2)])
+@pytest.mark.parametrize("nbeta,nseq", [(1, 5), (3, 7), (5, 13), (40, 3), (70, 2)])
 def test_lockstep_exchange_ladder_shapes(nbeta, nseq):
     # ladders of up to 32 rungs share a wave between 64 / 2^ceil(log2 nbeta)
-    # sequences (here with a partly filled last wave), 33..64 rungs take a
-    # wave per sequence and longer ladders the serial exchange
+    # sequences (here with a partly filled last wave; one rung: 64 sequences
+    # per wave and no swap attempts), 33..64 rungs take a wave per sequence
+    # and longer ladders the serial exchange
     d = synthdata.ring_gauss_dataset(nvis=200, ncph=60, seed=24)
     fit = eht_fit(d, nbeta=nbeta, nseq=nseq, nburn=2, niter=2, seed=31 + nbeta)
     compared, _ = _lockstep(fit, oracle_eht_chi2(d), 4)
