@@ -18,8 +18,8 @@
 //                    per lane and reduced across the wave
 //   k_step_quad      the same for systems of quadratic equations, one lane per
 //                    chain (wave-uniform coefficients)
-//   k_exchange       replica_exchange (:533-626); one wave per sequence, one
-//                    lane per temperature slot
+//   k_exchange       replica_exchange (:533-626); one lane per temperature
+//                    slot, 64 / xlanes sequences per wave
 #include <hip/hip_runtime.h>
 #include "mhre_device.h"
 #include "eht_model.h"

@@ -95,3 +95,27 @@ def test_autotune_keeps_chains_identical():
         s.run(total - 12)
         s.download()
     _same(ref, got)
+
+
+def _quad(n, seed, nb=8, ns=24, nburn=10, niter=10):
+    from cuda_mcmc_amd import quadsys
+    ds = synthdata.quadsys_dataset(nsys=1, neq=n, n=n, seed=seed)
+    return quadsys.quad_fit(ds["Q"][0], ds["b"][0], ds["c"][0], nbeta=nb, nseq=ns, nburn=nburn, niter=niter,
+                            seed=seed + 7, record=True)
+
+
+def test_mixed_kinds_in_one_session():
+    # ring+Gaussian fits of both models next to quadratic systems of two sizes:
+    # four step launches per iteration (one per kernel, own queue slots in the
+    # persistent form) and one exchange launch over ladders of 8, 5 and 12
+    # rungs; every fit must come out as it does alone
+    def make():
+        d = synthdata.ring_gauss_dataset(nvis=400, ncph=120, seed=91)
+        return [_fit(700, 200, 92, nb=8, ns=16, nburn=10, niter=10),
+                eht_fit(d, 5, 12, nburn=10, niter=10, seed=93, model="sgra13"),
+                _quad(8, 94, nb=12), _quad(4, 95)]
+    alone = [_run([f])[0] for f in make()]
+    for opt in (dict(persistent=0), dict(persistent=1), dict(persistent=1, graph_iters=4)):
+        together = _run(make(), **opt)
+        for a, b in zip(alone, together):
+            _same(a, b)
