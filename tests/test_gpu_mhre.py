@@ -149,3 +149,23 @@ def test_lockstep_exchange_ladder_shapes(nbeta, nseq):
     # sequences fall inside a near-tie band and are left out of the state
     # comparison; every decision outside those bands was still compared
     assert compared >= 2
+
+
+def test_lockstep_fixed_parameters():
+    # pdescr 0: ecc and the Gaussian axis ratio are held at their ptotal values
+    # (nprm 7 of nptot 9, ivar/invar maps), the orientation stays an angle
+    from cuda_mcmc_amd.engine import Fit
+    d = synthdata.ring_gauss_dataset(nvis=300, ncph=90, seed=25)
+    pdescr = synthdata.EHT_PDESCR.copy()
+    pdescr[3] = 0
+    pdescr[6] = 0
+    ptotal = (0.5 * (synthdata.EHT_PMIN + synthdata.EHT_PMAX)).astype(np.float32)
+    ptotal[3], ptotal[6] = 0.21, 0.8
+    idat = np.array([d["nvis"], 8 * 16 * (d["nvis"] + d["ncph"]), 0], np.int32)
+    fit = Fit(pdescr, synthdata.EHT_PMIN, synthdata.EHT_PMAX, d["coor"], d["dat"], d["std2r"], nbeta=8, nseq=16,
+              nburn=3, niter=3, model="sgra9", imodel=1, icoor=d["icoor"], idat=idat, ptotal=ptotal, npass=2,
+              seed=66)
+    assert fit.nprm == 7 and fit.nptot == 9
+    compared, _ = _lockstep(fit, oracle_eht_chi2(d), 6)
+    assert compared >= 6 * 16 * 0.5
+    assert np.all(fit.pcur[:, :, 3] == np.float32(0.21)) and np.all(fit.pcur[:, :, 6] == np.float32(0.8))
