@@ -329,11 +329,21 @@ int exchange_lanes(int nbeta) {
 int hist_words(int nadj) { return (nadj + 31) / 32; }
 
 // n_hist[a][q] (0/1, q = k * nchain + chain) -> bit a % 32 of word [a / 32][q]
-void pack_hist(const int* n_hist, int nadj, size_t npc, std::vector<uint32_t>& bits) {
+// Packs the 0/1 history n_hist[nadj][npc] into bits and, in the same pass,
+// the running accept sums sum[q] = number of set entries of q.
+void pack_hist(const int* n_hist, int nadj, size_t npc, std::vector<uint32_t>& bits, std::vector<int>& sum) {
   bits.assign((size_t)hist_words(nadj) * npc, 0u);
-  for (int a = 0; a < nadj; ++a)
-    for (size_t q = 0; q < npc; ++q)
-      if (n_hist[(size_t)a * npc + q]) bits[(size_t)(a >> 5) * npc + q] |= 1u << (a & 31);
+  sum.assign(npc, 0);
+  for (int a = 0; a < nadj; ++a) {
+    const int* h = n_hist + (size_t)a * npc;
+    uint32_t* w = bits.data() + (size_t)(a >> 5) * npc;
+    const uint32_t bit = 1u << (a & 31);
+    for (size_t q = 0; q < npc; ++q) {
+      const int on = h[q] != 0;
+      w[q] |= on ? bit : 0u;
+      sum[q] += on;
+    }
+  }
 }
 
 void unpack_hist(const std::vector<uint32_t>& bits, int nadj, size_t npc, int* n_hist) {
@@ -853,7 +863,9 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
     U.add(h.o_pstp, s.pstp, P * C);
     U.add(h.o_ncnt, s.n_cnt, P * C);
     U.add(h.o_nacpt, s.n_acpt, P * C);
-    pack_hist(s.n_hist, s.nadj, P * C, h.hbits);
+    // the history as bits and the running accept sums over its nadj entries
+    // (the device keeps sumh[k][chain] equal to the set bits of the chain's ring)
+    pack_hist(s.n_hist, s.nadj, P * C, h.hbits, h.sumh);
     U.add(h.o_nhist, h.hbits.data(), h.hbits.size());
     U.add(h.o_nexch, s.n_exch, C);
     U.add(h.o_tcur, s.tcur, C);
@@ -862,12 +874,14 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
     if (s.ptentn) U.add(h.o_ptentn, s.ptentn, C);
     if (h.has_datm && s.datm) U.add(h.o_datm, s.datm, (size_t)std::min<long long>(d.ndatm, s.ndatm));
     if (h.has_datm && s.chi2m) U.add(h.o_chi2m, s.chi2m, C * s.ndat);
-    // running accept sums over the nadj-deep history (kept exactly equal to
-    // sum_a n_hist[a][k][chain])
-    h.sumh.assign(P * C, 0);
-    for (int a = 0; a < s.nadj; ++a)
-      for (size_t q = 0; q < P * C; ++q) h.sumh[q] += s.n_hist[(size_t)a * P * C + q];
     U.add(h.o_sumh, h.sumh.data(), h.sumh.size());
+    if (h.record && it0 > s.nburn) {
+      // resumed inside the sampling stage: the samples recorded before it0
+      // come back unchanged with the download
+      U.add(h.o_pout, s.pout, P * C * (size_t)s.niter);
+      U.add(h.o_tout, s.tout, C * (size_t)s.niter);
+      U.add(h.o_chi2, s.chi2, (size_t)s.nseq * s.niter);
+    }
     CCalcModel mc{};
     mc.imodel = s.imodel; mc.ncoor = s.ncoor; mc.ndat = s.ndat; mc.nicoor = s.nicoor; mc.nidat = s.nidat;
     mc.ndatm = (int)std::min<long long>(d.ndatm, 0x7fffffff); mc.nptot = s.nptot; mc.nbeta = s.nbeta; mc.nseq = s.nseq;

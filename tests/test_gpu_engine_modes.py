@@ -119,3 +119,21 @@ def test_mixed_kinds_in_one_session():
         together = _run(make(), **opt)
         for a, b in zip(alone, together):
             _same(a, b)
+
+
+@pytest.mark.parametrize("split", [7, 15])
+def test_resume_in_a_second_session(split):
+    # stop after `split` iterations (inside burn-in, or inside sampling with
+    # samples already recorded), download, and continue in a new session from
+    # the downloaded state: the counter-based streams and the uploaded state
+    # give the run of one session, bit for bit
+    ref = _run([_fit(500, 150, 71, nburn=10, niter=10)])[0]
+    f = _fit(500, 150, 71, nburn=10, niter=10)
+    with Session([f]) as s:
+        s.run(split)
+        s.download()
+    f.it0, f.init_chains = split, False
+    with Session([f]) as s:
+        s.run(f.nburn + f.niter - split)
+        s.download()
+    _same(ref, f)
