@@ -169,3 +169,12 @@ def test_lockstep_fixed_parameters():
     compared, _ = _lockstep(fit, oracle_eht_chi2(d), 6)
     assert compared >= 6 * 16 * 0.5
     assert np.all(fit.pcur[:, :, 3] == np.float32(0.21)) and np.all(fit.pcur[:, :, 6] == np.float32(0.8))
+
+
+def test_lockstep_sgra13():
+    # the 13-parameter model: its float64 trigonometric invariants (alpha,
+    # beta, eta - th, th) are kept across proposals that move other parameters
+    d = synthdata.ring_gauss_dataset(nvis=300, ncph=90, seed=26)
+    fit = eht_fit(d, nbeta=8, nseq=16, nburn=3, niter=3, seed=88, model="sgra13")
+    compared, _ = _lockstep(fit, oracle_eht_chi2(d, imodel=2), 6)
+    assert compared >= 6 * 16 * 0.5
