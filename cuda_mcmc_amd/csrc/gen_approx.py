@@ -197,9 +197,6 @@ def main():
     eC = check32(mC, C, 0.0, 64.0)
     emit("BESSEL_B", mB, "J1(x)/x = sum c_k t^k, t = x^2/32 - 1, x in [0,8]; fp32 max abs err %.2e" % eB)
     emit("BESSEL_C", mC, "J2(x)/x^2 = sum c_k t^k, t = x^2/32 - 1, x in [0,8]; fp32 max abs err %.2e" % eC)
-    mD = cheb_monomial(D, 0.0, 64.0, 8)
-    eD = check32(mD, D, 0.0, 64.0)
-    emit("BESSEL_D", mD, "J3(x)/x^3 = sum c_k t^k, t = x^2/32 - 1, x in [0,8]; fp32 max abs err %.2e" % eD)
     nq = 257   # quarter-unit intervals of y covering [0, 64] (the last one only at y = 64)
     tb = piecewise_cubic(B, nq, h=0.25, deg=2)
     tc = piecewise_cubic(C, nq, h=0.25, deg=2)
@@ -213,6 +210,15 @@ def main():
     print("__device__ static constexpr float BESSEL_TAB[%d] = {" % tab.size)
     for j in range(nq):
         print("    " + " ".join("%.9ef," % v for v in tab[8 * j:8 * j + 8]))
+    print("};")
+    print()
+    td = piecewise_cubic(D, nq, h=0.25, deg=2)
+    ed = check_piecewise32(td, D, h=0.25)
+    tabd = np.concatenate([td, np.zeros((nq, 1))], axis=1).ravel()
+    print("// D on the same intervals: D0 D1 D2 0 per interval (gradient kernels); fp32 max abs err %.2e" % ed)
+    print("__device__ static constexpr float BESSEL_TABD[%d] = {" % tabd.size)
+    for j in range(nq):
+        print("    " + " ".join("%.9ef," % v for v in tabd[4 * j:4 * j + 4]))
     print("};")
     print()
     for n in (1, 2):

@@ -78,18 +78,37 @@ __device__ inline void sgra9_inv_dual(const Dual* p, Dual* out) {
 
 // ------------------------------------------------------ per-datum model
 
-// B, C and D at y = x^2 (polynomials for y <= 64; above, Hankel asymptotics
-// for B and C and D = (4C - B)/y from J3 = 4 J2/x - J1).
-__device__ inline void bessel_BCD(float y, float* B, float* C, float* D) {
-  if (y <= 64.0f) {
-    const float t = fmaf(y, 0.03125f, -1.0f);
-    *B = horner(BESSEL_B, t);
-    *C = horner(BESSEL_C, t);
-    *D = horner(BESSEL_D, t);
-  } else {
-    bessel_BC_asym(y, B, C);
-    *D = (4.0f * *C - *B) * __builtin_amdgcn_rcpf(y);
+// B, C and D at y = x^2.  y <= 64: the quarter-interval quadratics of
+// BESSEL_TAB (B, C; the fused kernels' table) and BESSEL_TABD (D), staged in
+// LDS by the kernel; above, Hankel asymptotics for B and C and
+// D = (4C - B)/y from J3 = 4 J2/x - J1.
+struct BesselTabBCD {
+  const float4* tab;    // 2 float4 per interval: B0 B1 B2 C0, C1 C2 - -
+  const float4* tabd;   // 1 float4 per interval: D0 D1 D2 -
+  __device__ void operator()(float y, float* B, float* C, float* D) const {
+    if (y <= 64.0f) {
+      const float z = 4.0f * y;
+      const int j = (int)z;
+      const float s = __builtin_amdgcn_fractf(z);
+      const float4 a = tab[2 * j], b = tab[2 * j + 1], c = tabd[j];
+      *B = fmaf(fmaf(a.z, s, a.y), s, a.x);
+      *C = fmaf(fmaf(b.y, s, b.x), s, a.w);
+      *D = fmaf(fmaf(c.z, s, c.y), s, c.x);
+    } else {
+      bessel_BC_asym(y, B, C);
+      *D = (4.0f * *C - *B) * __builtin_amdgcn_rcpf(y);
+    }
   }
+};
+
+// Copies BESSEL_TAB and BESSEL_TABD into the work-group's LDS tables (all
+// threads take part; call before any thread of the group leaves).
+__device__ inline void load_bessel_tabs(float4* tab, float4* tabd) {
+  float* t = reinterpret_cast<float*>(tab);
+  for (int i = (int)threadIdx.x; i < 8 * BESSEL_TAB_N; i += (int)blockDim.x) t[i] = BESSEL_TAB[i];
+  float* td = reinterpret_cast<float*>(tabd);
+  for (int i = (int)threadIdx.x; i < 4 * BESSEL_TAB_N; i += (int)blockDim.x) td[i] = BESSEL_TABD[i];
+  __syncthreads();
 }
 
 struct Fwd9 {
@@ -97,10 +116,10 @@ struct Fwd9 {
 };
 
 // Forward model of one visibility, keeping what the backward pass needs.
-// The model of Sgra9::vis with the full-range Bessel branch; phases go
-// through sincos_fast in radians (Sgra9::vis uses sincospi_fast on the same
-// phases in units of pi), and the ring term is mixed after the fact.
-__device__ inline Fwd9 sgra9_fwd(const Sgra9::Inv& q, float u, float v) {
+// The model of Sgra9::vis (same Bessel table, sin/cos of the phases in
+// units of pi) with the full-range Bessel branch; the ring term is mixed
+// after the fact, as the reverse pass needs the unmixed ring.
+__device__ inline Fwd9 sgra9_fwd(const Sgra9::Inv& q, float u, float v, const BesselTabBCD& bt) {
   Fwd9 f;
   f.u = u;
   f.v = v;
@@ -109,18 +128,18 @@ __device__ inline Fwd9 sgra9_fwd(const Sgra9::Inv& q, float u, float v) {
   f.rho2 = fmaf(f.U, f.U, f.W * f.W);
   f.ye = q.r.se * f.rho2;
   const float yi = q.r.rq2 * f.ye;
-  bessel_BCD(f.ye, &f.Be, &f.Ce, &f.De);
-  bessel_BCD(yi, &f.Bi, &f.Ci, &f.Di);
+  bt(f.ye, &f.Be, &f.Ce, &f.De);
+  bt(yi, &f.Bi, &f.Ci, &f.Di);
   const float ree = q.r.kBe * f.Be;
   const float ime = q.r.kCe * f.U * f.Ce;
   const float rei = q.r.kBi * f.Bi;
   const float imi = q.r.kCi * f.U * f.Ci;
-  sincos_fast(q.r.kd * f.U, &f.sd, &f.cd);
+  sincospi_fast(q.r.kdp * f.U, &f.sd, &f.cd);
   f.Fx = ree - (rei * f.cd - imi * f.sd);
   f.Fy = ime - (rei * f.sd + imi * f.cd);
   if (q.r.gauss) {
     f.ga = __builtin_amdgcn_exp2f(-fmaf(q.gaU, f.U * f.U, q.gaV * (f.W * f.W)));
-    sincos_fast(q.kg * f.U, &f.sg, &f.cgs);
+    sincospi_fast(q.kgp * f.U, &f.sg, &f.cgs);
     const float gw = q.r.cg * f.ga;
     f.Vx = fmaf(q.r.cr, f.Fx, gw * f.cgs);
     f.Vy = fmaf(q.r.cr, f.Fy, gw * f.sg);
@@ -228,6 +247,10 @@ __device__ inline void inv_jacobian(float pl, double* jac) {
 
 __global__ void __launch_bounds__(256) k_eht_grad9(GradArgs g) {
   __shared__ double jac_all[4][NINV * NP9];
+  __shared__ float4 btab[2 * BESSEL_TAB_N];
+  __shared__ float4 btabd[BESSEL_TAB_N];
+  load_bessel_tabs(btab, btabd);
+  const BesselTabBCD bt{btab, btabd};
   const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int w = blockIdx.x * 4 + wid;
   if (w >= g.n) return;
@@ -247,8 +270,8 @@ __global__ void __launch_bounds__(256) k_eht_grad9(GradArgs g) {
 #pragma unroll
   for (int m = 0; m < NINV; ++m) a[m] = 0.0;
   for (int i = lane; i < nvr; i += 64) {
-    const float4 d = f.vis[i];
-    const Fwd9 fw = sgra9_fwd(q, d.x, d.y);
+    const float4 d = gptr(f.vis)[i];
+    const Fwd9 fw = sgra9_fwd(q, d.x, d.y, bt);
     const float amp = __builtin_amdgcn_sqrtf(fmaf(fw.Vx, fw.Vx, fw.Vy * fw.Vy));
     const float r = d.z - amp;
     chi += (double)(r * r * d.w);
@@ -258,12 +281,13 @@ __global__ void __launch_bounds__(256) k_eht_grad9(GradArgs g) {
     }
   }
   for (int c = lane; c < f.ncph; c += 64) {
-    const int4 ix = f.clo[c];
-    const float4 ow = f.clow[c];
-    const float4 da = f.vis[ix.x], db = f.vis[ix.y], de = f.vis[ix.z];
-    const Fwd9 fa = sgra9_fwd(q, da.x, da.y);
-    const Fwd9 fb = sgra9_fwd(q, db.x, db.y);
-    const Fwd9 fe = sgra9_fwd(q, de.x, de.y);
+    const int4 ix = gptr(f.clo)[c];
+    const float4 ow = gptr(f.clow)[c];
+    const GLOBAL float4* vis = gptr(f.vis);
+    const float4 da = vis[ix.x], db = vis[ix.y], de = vis[ix.z];
+    const Fwd9 fa = sgra9_fwd(q, da.x, da.y, bt);
+    const Fwd9 fb = sgra9_fwd(q, db.x, db.y, bt);
+    const Fwd9 fe = sgra9_fwd(q, de.x, de.y, bt);
     const float dd = closure_resid(fa, fb, fe, ow);
     chi += (double)(dd * dd * ow.z);
     if (g.grad) {
@@ -311,6 +335,10 @@ __device__ inline void gn_accumulate(const float* rinv, float r, const double* j
 
 __global__ void __launch_bounds__(64) k_eht_gn9(GradArgs g) {
   __shared__ double jac[NINV * NP9];
+  __shared__ float4 btab[2 * BESSEL_TAB_N];
+  __shared__ float4 btabd[BESSEL_TAB_N];
+  load_bessel_tabs(btab, btabd);
+  const BesselTabBCD bt{btab, btabd};
   const int w = blockIdx.x;
   if (w >= g.n) return;
   const int lane = lane_id();
@@ -330,8 +358,8 @@ __global__ void __launch_bounds__(64) k_eht_gn9(GradArgs g) {
 #pragma unroll
   for (int i = 0; i < NJTJ; ++i) jtj[i] = 0.0;
   for (int i = lane; i < nvr; i += 64) {
-    const float4 d = f.vis[i];
-    const Fwd9 fw = sgra9_fwd(q, d.x, d.y);
+    const float4 d = gptr(f.vis)[i];
+    const Fwd9 fw = sgra9_fwd(q, d.x, d.y, bt);
     const float amp = __builtin_amdgcn_sqrtf(fmaf(fw.Vx, fw.Vx, fw.Vy * fw.Vy));
     const float sw = __builtin_amdgcn_sqrtf(d.w);
     const float r = (d.z - amp) * sw;
@@ -345,12 +373,13 @@ __global__ void __launch_bounds__(64) k_eht_gn9(GradArgs g) {
     gn_accumulate(rinv, r, jac, jtr, jtj);
   }
   for (int c = lane; c < f.ncph; c += 64) {
-    const int4 ix = f.clo[c];
-    const float4 ow = f.clow[c];
-    const float4 da = f.vis[ix.x], db = f.vis[ix.y], de = f.vis[ix.z];
-    const Fwd9 fa = sgra9_fwd(q, da.x, da.y);
-    const Fwd9 fb = sgra9_fwd(q, db.x, db.y);
-    const Fwd9 fe = sgra9_fwd(q, de.x, de.y);
+    const int4 ix = gptr(f.clo)[c];
+    const float4 ow = gptr(f.clow)[c];
+    const GLOBAL float4* vis = gptr(f.vis);
+    const float4 da = vis[ix.x], db = vis[ix.y], de = vis[ix.z];
+    const Fwd9 fa = sgra9_fwd(q, da.x, da.y, bt);
+    const Fwd9 fb = sgra9_fwd(q, db.x, db.y, bt);
+    const Fwd9 fe = sgra9_fwd(q, de.x, de.y, bt);
     const float sw = __builtin_amdgcn_sqrtf(ow.z);
     const float r = closure_resid(fa, fb, fe, ow) * sw;
     chi += (double)r * (double)r;

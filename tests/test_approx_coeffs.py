@@ -91,6 +91,13 @@ def test_bessel_table_piecewise_quadratics():
     Ct = np.where(x > 1e-3, jv(2, xs) / (xs * xs), 0.125 - y / 96.0)
     assert np.max(np.abs(B - Bt)) < 6e-8
     assert np.max(np.abs(C - Ct)) < 2e-8
+    # D(y) = J3(x)/x^3 for the gradient kernels, on the same intervals
+    tabd = a["BESSEL_TABD"].reshape(-1, 4)
+    assert tabd.shape[0] == tab.shape[0]
+    Dv = _horner32_rows(tabd[j, 0:3], s)
+    xd = np.maximum(x, 1e-2)
+    Dt = np.where(x > 1e-2, jv(3, xd) / (xd * xd * xd), 1.0 / 48.0 - y / 768.0)
+    assert np.max(np.abs(Dv - Dt)) < 5e-9
     # B(0) = 1/2 and C(0) = 1/8: the rho -> 0 limits of the reference model
     assert tab[0, 0] == pytest.approx(0.5, abs=1e-7) and tab[0, 3] == pytest.approx(0.125, abs=1e-7)
 
