@@ -145,6 +145,10 @@ __device__ double eht_chi2_wave(const EhtView& v, const typename M::Inv& q, floa
   int kleft = v.cgroup;
   int2 ti = ldg(v.tinfo, 1);
   float r2cur = __int_as_float(ldg(v.tinfo, 0).y);
+  // tile t needs the large-argument branch when se * r2 > 64, i.e. when r2
+  // exceeds 64 / se: a compare of the (non-negative) float bits, in scalar
+  // registers
+  const uint32_t r2_thr = __builtin_amdgcn_readfirstlane(__float_as_uint(64.0f / q.r.se));
   // tile t with its data d; fetches tile t + 1 into dn
   auto tile = [&](const int t, const float4& d, float4& dn) {
     const int i = t * TILE + lane;
@@ -153,7 +157,7 @@ __device__ double eht_chi2_wave(const EhtView& v, const typename M::Inv& q, floa
     const int cf = c0 + lane;
     int4 ix = ldg(v.clo, cf);
     float4 ow = ldg(v.clow, cf);
-    const bool any_large = q.r.se * r2cur > 64.0f;
+    const bool any_large = __float_as_uint(r2cur) > r2_thr;
     const float2 V = M::vis(q, d.x, d.y, any_large, bt);
     const float amp = __builtin_amdgcn_sqrtf(fmaf(V.x, V.x, V.y * V.y));
     const float r = d.z - amp;
