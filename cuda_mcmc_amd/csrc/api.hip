@@ -98,6 +98,9 @@ struct FitHost {
   std::vector<int> clo;          // 4 per closure
   std::vector<float> clow;       // 4 per closure
   std::vector<int> tinfo;        // 2 per tile: first closure, max u^2+v^2 (float bits)
+  std::vector<int> clr;          // gradient: leg rounds per closure (color_closure_legs)
+  int nround = 0;
+  bool ring_grad = false;        // the gradient may use the single-pass ring kernel
   std::vector<float> qcoef;
   std::vector<int> sumh;
   std::vector<uint32_t> hbits;   // packed acceptance history
@@ -1173,6 +1176,13 @@ MhreFitSpec legacy_spec(float* coor, float* dat, float* std2r, int* icoor, int* 
 
 // ----------------------------------------------- packed data for gradients
 
+// MHRE_GRAD_RING=0 selects the per-closure gradient kernel for every fit
+// (both forms compute the same gradient; the test suite compares them).
+static bool ring_grad_disabled() {
+  const char* e = std::getenv("MHRE_GRAD_RING");
+  return e && e[0] == '0';
+}
+
 struct EhtData {
   FitHost h;
   char* dev = nullptr;
@@ -1183,6 +1193,86 @@ void destroy_eht_data(EhtData* E) {
   if (!E) return;
   if (E->dev) hipFree(E->dev);
   delete E;
+}
+
+// The ring gradient kernel adds each closure's adjoint to its three
+// visibilities' slots in LDS.  Float atomics there are slow, so the legs of
+// every 64-closure batch (the kernel's batches: cgroup tiles' closures, 64 at
+// a time) are split into rounds in which no two legs name the same
+// visibility; a round is then a plain read-add-write.  The rounds are a
+// proper edge colouring of the bipartite multigraph lanes x visibilities,
+// built with Koenig's alternating-path step, so a batch needs only as many
+// rounds as its largest degree (3, or the most closures sharing one
+// visibility).  More than 16 rounds leaves the data to the per-closure kernel.
+static void color_closure_legs(FitHost& h) {
+  constexpr int MAXR = 16;
+  const int ncph = h.ncph;
+  h.clr.assign((size_t)ncph + TILE, 0);
+  h.nround = 0;
+  h.ring_grad = false;
+  if (!h.ring_ok || ncph == 0) return;
+  std::vector<int> at, color, vnode;
+  std::vector<int> vis_ids;
+  for (int t0 = 0, c0 = 0; t0 < h.ntile; t0 += h.cgroup) {
+    const int t1 = std::min(t0 + h.cgroup, h.ntile);
+    const int c1 = h.tinfo[2 * t1];
+    for (int b0 = c0; b0 < c1; b0 += TILE) {
+      const int nb = std::min(TILE, c1 - b0);
+      // right-hand nodes: the batch's distinct visibilities, after the TILE lanes
+      vis_ids.clear();
+      vnode.assign(3 * nb, 0);
+      for (int e = 0; e < 3 * nb; ++e) {
+        const int v = h.clo[4 * (b0 + e / 3) + e % 3];
+        int k = (int)(std::find(vis_ids.begin(), vis_ids.end(), v) - vis_ids.begin());
+        if (k == (int)vis_ids.size()) vis_ids.push_back(v);
+        vnode[e] = TILE + k;
+      }
+      const int nnode = TILE + (int)vis_ids.size();
+      at.assign((size_t)nnode * MAXR, -1);   // at[node * MAXR + c]: edge of colour c there
+      color.assign(3 * nb, -1);
+      auto other = [&](int e, int x) { return x == e / 3 ? vnode[e] : e / 3; };
+      auto free_col = [&](int x) {
+        int c = 0;
+        while (c < MAXR && at[(size_t)x * MAXR + c] >= 0) ++c;
+        return c;
+      };
+      for (int e = 0; e < 3 * nb; ++e) {
+        const int u = e / 3, v = vnode[e];
+        const int a = free_col(u), b = free_col(v);
+        if (a == MAXR || b == MAXR) return;   // degree above MAXR
+        if (at[(size_t)v * MAXR + a] >= 0) {
+          // swap a <-> b along the path from v that alternates a, b; in a
+          // bipartite graph it cannot reach u, so a stays free at u
+          std::vector<int> path;
+          int x = v, c = a;
+          while (at[(size_t)x * MAXR + c] >= 0) {
+            const int f = at[(size_t)x * MAXR + c];
+            path.push_back(f);
+            x = other(f, x);
+            c = c == a ? b : a;
+          }
+          for (int f : path) {
+            at[(size_t)(f / 3) * MAXR + color[f]] = -1;
+            at[(size_t)vnode[f] * MAXR + color[f]] = -1;
+          }
+          for (int f : path) {
+            color[f] = color[f] == a ? b : a;
+            at[(size_t)(f / 3) * MAXR + color[f]] = f;
+            at[(size_t)vnode[f] * MAXR + color[f]] = f;
+          }
+        }
+        color[e] = a;
+        at[(size_t)u * MAXR + a] = e;
+        at[(size_t)v * MAXR + a] = e;
+      }
+      for (int j = 0; j < nb; ++j) {
+        h.clr[b0 + j] = color[3 * j] | color[3 * j + 1] << 4 | color[3 * j + 2] << 8;
+        for (int m = 0; m < 3; ++m) h.nround = std::max(h.nround, color[3 * j + m] + 1);
+      }
+    }
+    c0 = c1;
+  }
+  h.ring_grad = true;
 }
 
 int eht_data_create(const float* coor, const float* dat, const float* std2r, const int* icoor, const int* idat,
@@ -1207,11 +1297,13 @@ int eht_data_create(const float* coor, const float* dat, const float* std2r, con
   s.ncoor = ncoor; s.ndat = ndat; s.nicoor = nicoor; s.nidat = nidat; s.nptot = nptot;
   h.kind = K_SGRA9;
   CHK(pack_eht(h, 0));
+  color_closure_legs(h);
   Arena A;
   const size_t o_vis = A.reserve<float>(h.vis.size());
   const size_t o_clo = A.reserve<int>(h.clo.size());
   const size_t o_clow = A.reserve<float>(h.clow.size());
   const size_t o_tinfo = A.reserve<int>(h.tinfo.size());
+  const size_t o_clr = A.reserve<int>(h.clr.size());
   const size_t o_fd = A.reserve<FitDesc>(1);
   HIPCHK(hipMalloc((void**)&E->dev, A.bytes));
   FitDesc d{};
@@ -1224,11 +1316,14 @@ int eht_data_create(const float* coor, const float* dat, const float* std2r, con
   d.clo = reinterpret_cast<const int4*>(E->dev + o_clo);
   d.clow = reinterpret_cast<const float4*>(E->dev + o_clow);
   d.tinfo = reinterpret_cast<const int2*>(E->dev + o_tinfo);
+  d.clr = reinterpret_cast<const int*>(E->dev + o_clr);
+  d.nround = h.nround;
   E->d_fd = reinterpret_cast<FitDesc*>(E->dev + o_fd);
   HIPCHK(hipMemcpy(E->dev + o_vis, h.vis.data(), h.vis.size() * sizeof(float), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(E->dev + o_clo, h.clo.data(), h.clo.size() * sizeof(int), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(E->dev + o_clow, h.clow.data(), h.clow.size() * sizeof(float), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(E->dev + o_tinfo, h.tinfo.data(), h.tinfo.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(E->dev + o_clr, h.clr.data(), h.clr.size() * sizeof(int), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(E->d_fd, &d, sizeof(d), hipMemcpyHostToDevice));
   h.d = d;
   // the caller's arrays are not referenced after this point
@@ -1456,7 +1551,8 @@ int mhre_eht_chi2_grad(void* handle, const float* params, int n, int ldp, double
   (void)hipGetLastError();
   const EhtData* E = static_cast<const EhtData*>(handle);
   CHK(eht_check_batch(E, params, n, ldp, chi2));
-  HIPCHK(launch_eht_grad9(E->d_fd, params, n, ldp, chi2, grad, static_cast<hipStream_t>(stream)));
+  const int ring = E->h.ring_grad && !ring_grad_disabled();
+  HIPCHK(launch_eht_grad9(E->d_fd, params, n, ldp, chi2, grad, ring, static_cast<hipStream_t>(stream)));
   return MHRE_OK;
 }
 

@@ -61,6 +61,10 @@ struct FitDesc {
   const int2* tinfo;       // [ntile + 2] per tile t: first closure whose largest index
                            // lies past tile t - 1, and the max u^2+v^2 of tile t's
                            // visibilities (float bits)
+  const int* clr;          // [ncph + TILE] gradient only: round of each closure leg
+                           // (4 bits per leg), no two legs of one 64-closure batch
+                           // and round naming the same visibility; null elsewhere
+  int nround;              // rounds per batch (gradient only)
   const float* qcoef;      // MODEL_QUADSYS packed coefficients
   const float* coor; const float* dat; const float* std2r;
   const int* icoor; const int* idat;

@@ -7,8 +7,10 @@ namespace mhre {
 
 // chi2[n] (and grad[n][ldp] when grad != nullptr) of the 9-parameter model
 // for the parameter vectors P[n][ldp] against one fit's packed data fd.
+// ring != 0: the fit's closures lie inside the ring window (FitDesc::ring_ok),
+// so the single-pass ring form is used.
 hipError_t launch_eht_grad9(const FitDesc* fd, const float* P, int n, int ldp, double* chi2, float* grad,
-                            hipStream_t s);
+                            int ring, hipStream_t s);
 // chi2[n], J^T r [n][9] and the upper triangle of J^T J [n][45] (row-major)
 // of the weighted residuals r.
 hipError_t launch_eht_gn9(const FitDesc* fd, const float* P, int n, int ldp, double* chi2, double* jtr,

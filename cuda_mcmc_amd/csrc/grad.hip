@@ -314,6 +314,160 @@ __global__ void __launch_bounds__(256) k_eht_grad9(GradArgs g) {
   }
 }
 
+// Closure residual from the model visibilities alone (closure_resid).
+__device__ inline float closure_resid_v(float2 a, float2 b, float2 e, float4 ow) {
+  const float abr = a.x * b.x - a.y * b.y, abi = a.x * b.y + a.y * b.x;
+  const float br = abr * e.x + abi * e.y, bi = abi * e.x - abr * e.y;
+  const float xr = ow.x * br + ow.y * bi;
+  const float xi = ow.y * br - ow.x * bi;
+  return atan2_fast(xi, xr);
+}
+
+// Gradient for fits whose closures lie inside the 512-visibility ring window
+// (FitDesc::ring_ok).  One wave per parameter point streams the 64-wide
+// tiles once, forming the closures after every cgroup tiles from an LDS ring
+// of model visibilities, as the step kernel does.  d(arg V)/dV is linear in
+// its scale, so the closure terms of a visibility reduce to one scalar
+// S_j = sum of +-(-2 dd w) over the closures that contain it; S_j is
+// accumulated in a second ring, in FitDesc::nround collision-free rounds of
+// plain read-add-writes (FitDesc::clr).  A visibility's reverse pass (amplitude term
+// and darg(V_j, S_j) together) runs once no later closure group can reach it,
+// just before its ring slot is reused, with its forward recomputed.  Per
+// visibility that is two forward and one reverse pass, where k_eht_grad9 runs
+// three of each for every closure.
+static_assert(MAX_CGROUP <= RING / TILE, "a closure group must fit in the ring");
+__global__ void __launch_bounds__(256) k_eht_grad9_ring(GradArgs g) {
+  __shared__ double jac_all[4][NINV * NP9];
+  __shared__ float2 ringv_all[4][RING];
+  __shared__ float rings_all[4][RING];
+  __shared__ float4 btab[2 * BESSEL_TAB_N];
+  __shared__ float4 btabd[BESSEL_TAB_N];
+  load_bessel_tabs(btab, btabd);
+  const BesselTabBCD bt{btab, btabd};
+  const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int w = blockIdx.x * 4 + wid;
+  if (w >= g.n) return;
+  const int lane = lane_id();
+  const FitDesc& f = *g.fd;
+  const float pl = lane < NP9 ? g.P[(size_t)w * g.ldp + lane] : 0.0f;
+  float p[NP9];
+#pragma unroll
+  for (int j = 0; j < NP9; ++j) p[j] = bcast(pl, j);
+  const Sgra9::Inv q = Sgra9::prepare(p);
+  double* jac = jac_all[wid];
+  float2* ringv = ringv_all[wid];
+  float* rings = rings_all[wid];
+  const bool grad = g.grad != nullptr;
+  if (grad) inv_jacobian(pl, jac);
+
+  const GLOBAL float4* vis = gptr(f.vis);
+  const GLOBAL int4* clo = gptr(f.clo);
+  const GLOBAL float4* clow = gptr(f.clow);
+  const GLOBAL int2* tinfo = gptr(f.tinfo);
+  const GLOBAL int* clr = gptr(f.clr);
+  const int nvr = f.ndat - f.ncph, nvis = f.nvis, ntile = f.ntile, cg = f.cgroup, nround = f.nround;
+  constexpr int RT = RING / TILE;   // tiles the ring holds
+  double chi = 0.0;
+  double a[NINV];
+#pragma unroll
+  for (int m = 0; m < NINV; ++m) a[m] = 0.0;
+  // reverse pass of tile tb, whose closures have all been formed
+  auto reverse = [&](int tb, float4 d) {
+    const int j = tb * TILE + lane;
+    if (j < nvis) {
+      const Fwd9 fw = sgra9_fwd(q, d.x, d.y, bt);
+      const float m2 = fmaxf(fmaf(fw.Vx, fw.Vx, fw.Vy * fw.Vy), 1e-30f);
+      float sa = 0.0f;
+      if (j < nvr) {
+        const float amp = __builtin_amdgcn_sqrtf(m2);
+        sa = -2.0f * (d.z - amp) * d.w * __builtin_amdgcn_rcpf(amp);
+      }
+      // amplitude: s (Vx, Vy); closures: darg(V, S) = S (-Vy, Vx) / |V|^2
+      const float k = rings[j & (RING - 1)] * __builtin_amdgcn_rcpf(m2);
+      sgra9_bwd(q, fw, fmaf(sa, fw.Vx, -k * fw.Vy), fmaf(sa, fw.Vy, k * fw.Vx), a);
+    }
+  };
+  int c0 = 0, rnext = 0;
+  float4 dnext = vis[lane];
+  for (int t0 = 0; t0 < ntile; t0 += cg) {
+    const int t1 = t0 + cg < ntile ? t0 + cg : ntile;
+    for (int t = t0; t < t1; ++t) {
+      const int i = t * TILE + lane;
+      const float4 d = dnext;
+      dnext = vis[i + TILE];   // padded by one zero-weight tile
+      const Fwd9 fw = sgra9_fwd(q, d.x, d.y, bt);
+      if (i < nvr) {
+        const float r = d.z - __builtin_amdgcn_sqrtf(fmaf(fw.Vx, fw.Vx, fw.Vy * fw.Vy));
+        chi += (double)(r * r * d.w);
+      }
+      ringv[i & (RING - 1)] = make_float2(fw.Vx, fw.Vy);
+      rings[i & (RING - 1)] = 0.0f;
+    }
+    wave_sync_lds();
+    // closures whose largest visibility index lies in tiles [t0, t1)
+    const int c1 = tinfo[t1].x;
+    for (int cb = c0; cb < c1; cb += TILE) {
+      const int c = cb + lane;
+      if (c < c1) {
+        const int4 ix = clo[c];
+        const float4 ow = clow[c];
+        const float2 va = ringv[ix.x & (RING - 1)], vb = ringv[ix.y & (RING - 1)], ve = ringv[ix.z & (RING - 1)];
+        const float dd = closure_resid_v(va, vb, ve, ow);
+        chi += (double)(dd * dd * ow.z);
+        if (grad) {
+          // dd = obs - pa - pb + pc: d(dd^2 w)/dpa = -2 dd w
+          const float s = -2.0f * dd * ow.z;
+          const int code = clr[c];
+          const int ra = code & 15, rb = (code >> 4) & 15, re = (code >> 8) & 15;
+          for (int r = 0; r < nround; ++r) {
+            // at most one leg of this closure, and no other lane's leg of
+            // this round, names a given visibility
+            if (r == ra || r == rb || r == re) {
+              const int v = r == ra ? ix.x : r == rb ? ix.y : ix.z;
+              rings[v & (RING - 1)] += r == re ? -s : s;
+            }
+            wave_sync_lds();
+          }
+        }
+      }
+    }
+    c0 = c1;
+    wave_sync_lds();
+    // The next group [t1, t2) forms its closures after tile t2 - 1, so they
+    // reach back to tile t2 - RT at the earliest, and its tiles reuse the
+    // slots of tiles t1 - RT .. t2 - 1 - RT: those are final now, and
+    // computed (t2 - t1 <= cg <= RT).
+    if (grad) {
+      const int rend = (t1 + cg < ntile ? t1 + cg : ntile) - RT;
+      if (rnext < rend) {
+        float4 dr = vis[rnext * TILE + lane];
+        for (; rnext < rend; ++rnext) {
+          const float4 d = dr;
+          if (rnext + 1 < rend) dr = vis[(rnext + 1) * TILE + lane];
+          reverse(rnext, d);
+        }
+      }
+    }
+  }
+  if (grad) {
+    wave_sync_lds();
+    for (; rnext < ntile; ++rnext) reverse(rnext, vis[rnext * TILE + lane]);
+  }
+  chi = wave_sum(chi);
+  if (lane == 0) g.chi2[w] = chi;
+  if (!grad) return;
+#pragma unroll
+  for (int m = 0; m < NINV; ++m) a[m] = wave_sum(a[m]);
+  if (lane < g.ldp) {
+    double gj = 0.0;
+    if (lane < NP9) {
+#pragma unroll
+      for (int m = 0; m < NINV; ++m) gj += a[m] * jac[m * NP9 + lane];
+    }
+    g.grad[(size_t)w * g.ldp + lane] = (float)gj;
+  }
+}
+
 // Adds the residual row of one datum: dr/dp = J_inv^T dr/dinv.
 __device__ inline void gn_accumulate(const float* rinv, float r, const double* jac, double* jtr, double* jtj) {
   float row[NP9];
@@ -408,10 +562,11 @@ __global__ void __launch_bounds__(64) k_eht_gn9(GradArgs g) {
 }
 
 hipError_t launch_eht_grad9(const FitDesc* fd, const float* P, int n, int ldp, double* chi2, float* grad,
-                            hipStream_t s) {
+                            int ring, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   GradArgs g{fd, P, n, ldp, chi2, grad, nullptr, nullptr};
-  k_eht_grad9<<<(n + 3) / 4, 256, 0, s>>>(g);
+  if (ring) k_eht_grad9_ring<<<(n + 3) / 4, 256, 0, s>>>(g);
+  else k_eht_grad9<<<(n + 3) / 4, 256, 0, s>>>(g);
   return hipGetLastError();
 }
 

@@ -193,7 +193,10 @@ int mhre_eht_data_create(const float *coor, const float *dat, const float *std2r
                          int ncoor, int ndat, int nicoor, int nidat, int nptot,
                          void **handle);
 /* chi2[n] and, if grad != NULL, grad[n][ldp] = d chi^2 / d params for the
- * parameter vectors params[n][ldp] (9 <= ldp <= 32; entries past 9 get 0). */
+ * parameter vectors params[n][ldp] (9 <= ldp <= 32; entries past 9 get 0).
+ * Data whose closures lie within the 512-visibility ring window use the
+ * single-pass ring kernel; the environment variable MHRE_GRAD_RING=0 selects
+ * the per-closure kernel instead (same results to float rounding). */
 int mhre_eht_chi2_grad(void *handle, const float *params, int n, int ldp,
                        double *chi2, float *grad, void *stream);
 /* Gauss-Newton normal equations of the weighted residuals r (amplitudes:

@@ -161,3 +161,24 @@ def test_laplace_vs_chain_covariance():
     ratio = sd_mc / sd_lap
     print("sd chain", sd_mc, "sd laplace", sd_lap, "ratio", ratio)
     assert np.all((ratio > 1 / 1.35) & (ratio < 1.35)), ratio
+
+
+@pytest.mark.parametrize("cpext,nsta", [(0, 8), (1, 8), (0, 12)])
+def test_ring_and_per_closure_gradient_forms_agree(monkeypatch, cpext, nsta):
+    # time-ordered data use the single-pass ring form (adjoints accumulated
+    # per visibility in collision-free rounds); MHRE_GRAD_RING=0 forces the
+    # per-closure form, which recomputes the three visibilities of every
+    # closure.  12 stations put one baseline in up to 10 closures of a batch.
+    d = synthdata.ring_gauss_dataset(nvis=1400, ncph=600, nsta=nsta, seed=19)
+    if cpext:   # closures from their own (u1, v1, u2, v2), appended after the visibility coordinates
+        c = d["cpix"]
+        uc = np.stack([d["u"][c[:, 0]], d["v"][c[:, 0]], d["u"][c[:, 1]], d["v"][c[:, 1]]], 1)
+        d = dict(d, coor=np.concatenate([d["coor"], uc.ravel()]).astype(np.float32), icoor=None, cpext=1)
+    P = _inner_prior(np.random.default_rng(13), 40)
+    _, chi_r, g_r = _gpu_grad(d, P)
+    monkeypatch.setenv("MHRE_GRAD_RING", "0")
+    _, chi_c, g_c = _gpu_grad(d, P)
+    np.testing.assert_allclose(chi_r, chi_c, rtol=1e-6)
+    err = np.linalg.norm(g_r - g_c, axis=1) / np.linalg.norm(g_c, axis=1)
+    print("ring vs per-closure max row difference", err.max())
+    assert np.all(err < 1e-5)
