@@ -21,8 +21,10 @@ Also reported:
                 session at a time (fused native kernel, and the per-datum
                 model.cuh contract path);
   adjoint       chi^2 + gradient of 4096 parameter points against 20000
-                visibilities + 6000 closure phases (fp32, HIP events), and a
-                Gauss-Newton polish + Laplace covariance of short fits;
+                visibilities + 6000 closure phases (fp32, HIP events), then
+                64 short fits of the plain-fit shape sampled in one batched
+                session and a Gauss-Newton polish + Laplace covariance of
+                each fit's best chain;
   quadsys       1024 systems of 8 quadratic equations, 16 x 32 chains each
                 (524288 chains), 3000 + 10000 iterations in one session, then
                 Gauss-Newton polish; success = residual norm < 1e-5.
@@ -170,7 +172,7 @@ def batched_vs_plain(iters, nfits, log_every=16):
                 pct_fp32_peak_batched=100.0 * batched * F_VIS / (PEAK_FP32_TFLOPS * 1e12))
 
 
-def adjoint_leg(npts=4096, nvis=20000, ncph=6000, reps=5, gn_fits=8):
+def adjoint_leg(npts=4096, nvis=20000, ncph=6000, reps=5, gn_fits=64, gn_burn=1000, gn_iter=250):
     """Config 4: batched chi^2 gradient throughput, then GN polish + Laplace."""
     import torch
     from cuda_mcmc_amd import adjoint, synthdata
@@ -198,14 +200,18 @@ def adjoint_leg(npts=4096, nvis=20000, ncph=6000, reps=5, gn_fits=8):
     out = dict(points=npts, nvis=nvis, ncph=ncph, ms_per_call=round(ms, 4), ms_chi2_only=round(ms_fwd, 4),
                vis_grad_evals_per_s=npts * nvis / (ms * 1e-3),
                model_evals_per_s=npts * (nvis + 3 * ncph) / (ms * 1e-3))
-    # short fits of the config-2 shape, then Gauss-Newton on each best chain
+    # 64 short fits of the config-2 shape sampled in one batched session,
+    # then Gauss-Newton on the best chain of each and its Laplace covariance
     before, after, ratios = [], [], []
-    for f in range(gn_fits):
-        dd = synthdata.ring_gauss_dataset(nvis=2000, ncph=600, nsta=8, seed=500 + f)
-        fit = eht_fit(dd, 32, 128, nburn=600, niter=400, seed=500 + f)
-        with Session([fit]) as s:
-            s.run(fit.nburn + fit.niter)
-            s.download()
+    dsets = [synthdata.ring_gauss_dataset(nvis=2000, ncph=600, nsta=8, seed=500 + f) for f in range(gn_fits)]
+    fits = [eht_fit(dd, 32, 128, nburn=gn_burn, niter=gn_iter, seed=500 + f) for f, dd in enumerate(dsets)]
+    t0 = time.time()
+    with Session(fits) as s:
+        s.run(gn_burn + gn_iter)
+        s.download()
+    t_sample = time.time() - t0
+    t0 = time.time()
+    for dd, fit in zip(dsets, fits):
         i = np.unravel_index(np.argmin(fit.chi2c), fit.chi2c.shape)
         dat = adjoint.EhtData.from_dataset(dd)
         p, c = adjoint.gauss_newton(dat, fit.pcur[i][None, :], synthdata.EHT_PMIN, synthdata.EHT_PMAX,
@@ -217,7 +223,11 @@ def adjoint_leg(npts=4096, nvis=20000, ncph=6000, reps=5, gn_fits=8):
         smp[:, 8] = np.angle(np.exp(1j * (smp[:, 8] - p[0, 8]))) + p[0, 8]
         sd_mc = smp[:, vv].std(axis=0)
         ratios.append(float(np.median(sd_mc / np.sqrt(np.diag(cov[0])))))
-    out["gauss_newton"] = dict(fits=gn_fits, iterations=20, chi2_best_chain_median=float(np.median(before)),
+        dat.close()
+    t_polish = time.time() - t0
+    out["gauss_newton"] = dict(fits=gn_fits, burn_in=gn_burn, samples=gn_iter, sampling_s=round(t_sample, 2),
+                               polish_and_laplace_s=round(t_polish, 2), iterations=20,
+                               chi2_best_chain_median=float(np.median(before)),
                                chi2_polished_median=float(np.median(after)),
                                chi2_drop_median=float(np.median(np.array(before) - np.array(after))),
                                laplace_vs_chain_sd_ratio_median=float(np.median(ratios)))
