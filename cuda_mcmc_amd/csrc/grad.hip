@@ -368,9 +368,22 @@ __global__ void __launch_bounds__(256) k_eht_grad9_ring(GradArgs g) {
   const int nvr = f.ndat - f.ncph, nvis = f.nvis, ntile = f.ntile, cg = f.cgroup, nround = f.nround;
   constexpr int RT = RING / TILE;   // tiles the ring holds
   double chi = 0.0;
+  // the invariants' adjoints: float partial sums over a few reverse passes
+  // (one group's, or the last ring's: at most RT tiles), then fp64 totals
   double a[NINV];
+  float af[NINV];
 #pragma unroll
-  for (int m = 0; m < NINV; ++m) a[m] = 0.0;
+  for (int m = 0; m < NINV; ++m) {
+    a[m] = 0.0;
+    af[m] = 0.0f;
+  }
+  auto flush = [&]() {
+#pragma unroll
+    for (int m = 0; m < NINV; ++m) {
+      a[m] += (double)af[m];
+      af[m] = 0.0f;
+    }
+  };
   // reverse pass of tile tb, whose closures have all been formed
   auto reverse = [&](int tb, float4 d) {
     const int j = tb * TILE + lane;
@@ -384,7 +397,7 @@ __global__ void __launch_bounds__(256) k_eht_grad9_ring(GradArgs g) {
       }
       // amplitude: s (Vx, Vy); closures: darg(V, S) = S (-Vy, Vx) / |V|^2
       const float k = rings[j & (RING - 1)] * __builtin_amdgcn_rcpf(m2);
-      sgra9_bwd(q, fw, fmaf(sa, fw.Vx, -k * fw.Vy), fmaf(sa, fw.Vy, k * fw.Vx), a);
+      sgra9_bwd(q, fw, fmaf(sa, fw.Vx, -k * fw.Vy), fmaf(sa, fw.Vy, k * fw.Vx), af);
     }
   };
   int c0 = 0, rnext = 0;
@@ -446,12 +459,14 @@ __global__ void __launch_bounds__(256) k_eht_grad9_ring(GradArgs g) {
           if (rnext + 1 < rend) dr = vis[(rnext + 1) * TILE + lane];
           reverse(rnext, d);
         }
+        flush();
       }
     }
   }
   if (grad) {
     wave_sync_lds();
     for (; rnext < ntile; ++rnext) reverse(rnext, vis[rnext * TILE + lane]);
+    flush();
   }
   chi = wave_sum(chi);
   if (lane == 0) g.chi2[w] = chi;
