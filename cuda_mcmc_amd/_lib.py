@@ -158,6 +158,8 @@ def _declare(L):
     L.mhre_eht_gn_system.argtypes = [vp, vp, C.c_int, C.c_int, vp, vp, vp, vp]
     L.mhre_eht_data_destroy.restype = C.c_int
     L.mhre_eht_data_destroy.argtypes = [vp]
+    L.mhre_eht_data_info.restype = C.c_int
+    L.mhre_eht_data_info.argtypes = [vp, C.POINTER(C.c_int), C.c_int]
     L.mhre_quad_jacobian.restype = C.c_int
     L.mhre_quad_jacobian.argtypes = [fp, fp, fp, ip, C.c_int, C.c_int, C.c_int, fp, C.c_int, fp, fp]
 

@@ -72,6 +72,12 @@ class EhtData:
         idat = np.array([d["nvis"], 0, int(d.get("cpext", 0))], np.int32)
         return cls(d["coor"], d["dat"], d["std2r"], d["icoor"], idat, **kw)
 
+    def info(self):
+        """How the data were packed: dict(ring_gradient, rounds, cgroup, ring_ok)."""
+        out = (C.c_int * 4)()
+        _lib.check(self.lib, self.lib.mhre_eht_data_info(self.handle, out, 4), "mhre_eht_data_info")
+        return dict(ring_gradient=bool(out[0]), rounds=int(out[1]), cgroup=int(out[2]), ring_ok=bool(out[3]))
+
     def close(self):
         if getattr(self, "handle", None) is not None and self.handle.value:
             self.lib.mhre_eht_data_destroy(self.handle)

@@ -1571,6 +1571,17 @@ int mhre_eht_data_destroy(void* handle) {
   return MHRE_OK;
 }
 
+int mhre_eht_data_info(const void* handle, int* info, int ninfo) {
+  if (!handle || !info) FAIL(MHRE_EINVAL, "null handle or info");
+  if (ninfo < 4) FAIL(MHRE_EINVAL, "ninfo = %d, need 4", ninfo);
+  const FitHost& h = static_cast<const EhtData*>(handle)->h;
+  info[0] = h.ring_grad ? 1 : 0;
+  info[1] = h.nround;
+  info[2] = h.cgroup;
+  info[3] = h.ring_ok;
+  return MHRE_OK;
+}
+
 int mhre_quad_jacobian(const float* Q, const float* b, const float* dat, const int* sys, int nsys, int neq, int n,
                        const float* X, int rows, float* r, float* J) {
   (void)hipGetLastError();

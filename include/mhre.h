@@ -206,6 +206,11 @@ int mhre_eht_chi2_grad(void *handle, const float *params, int n, int ldp,
 int mhre_eht_gn_system(void *handle, const float *params, int n, int ldp,
                        double *chi2, double *jtr, double *jtj, void *stream);
 int mhre_eht_data_destroy(void *handle);
+/* How the data were packed (no reference counterpart): info[0] = 1 if the
+ * gradient uses the single-pass ring kernel, info[1] = its closure-adjoint
+ * rounds per 64-closure batch, info[2] = tiles per closure group (cgroup),
+ * info[3] = 1 if every closure lies in the ring window.  ninfo >= 4. */
+int mhre_eht_data_info(const void *handle, int *info, int ninfo);
 
 /* Residuals r[rows][neq] = x^T Q_e x + b_e . x - dat_e and Jacobian
  * J[rows][neq][n] of the quadratic systems Q[nsys][neq][n][n],

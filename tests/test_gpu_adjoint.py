@@ -182,3 +182,30 @@ def test_ring_and_per_closure_gradient_forms_agree(monkeypatch, cpext, nsta):
     err = np.linalg.norm(g_r - g_c, axis=1) / np.linalg.norm(g_c, axis=1)
     print("ring vs per-closure max row difference", err.max())
     assert np.all(err < 1e-5)
+
+
+def test_gradient_kernel_selection_and_fallback():
+    # time-ordered data qualify for the ring kernel; the same data in a random
+    # visibility order spread closures over the whole set and fall back to the
+    # per-closure kernel, with the same chi^2 and gradient
+    d = synthdata.ring_gauss_dataset(nvis=1400, ncph=600, nsta=8, seed=23)
+    info = adjoint.EhtData.from_dataset(d).info()
+    print("time-ordered", info)
+    assert info["ring_gradient"] and info["ring_ok"] and 3 <= info["rounds"] <= 16
+    nv = d["nvis"]
+    perm = np.random.default_rng(3).permutation(nv)
+    inv = np.argsort(perm)
+    ds = dict(d, coor=np.ascontiguousarray(d["coor"].reshape(nv, 2)[perm].ravel()),
+              dat=np.concatenate([d["dat"][:nv][perm], d["dat"][nv:]]),
+              std2r=np.concatenate([d["std2r"][:nv][perm], d["std2r"][nv:]]),
+              icoor=inv[d["icoor"]].astype(np.int32))
+    info_s = adjoint.EhtData.from_dataset(ds).info()
+    print("shuffled", info_s)
+    assert not info_s["ring_gradient"] and not info_s["ring_ok"]
+    P = _inner_prior(np.random.default_rng(29), 24)
+    _, chi_a, g_a = _gpu_grad(d, P)
+    _, chi_b, g_b = _gpu_grad(ds, P)
+    np.testing.assert_allclose(chi_b, chi_a, rtol=1e-5)
+    err = np.linalg.norm(g_b - g_a, axis=1) / np.linalg.norm(g_a, axis=1)
+    print("shuffled vs time-ordered max row difference", err.max())
+    assert np.all(err < 1e-5)
