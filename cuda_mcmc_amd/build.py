@@ -93,6 +93,16 @@ def build_pmc_copy(verbose=True):
     return out
 
 
+def build_sincos_check(verbose=True):
+    """The accuracy check of the hardware sine/cosine (cuda_mcmc_amd/lib/sincos_check)."""
+    out = os.path.join(LIBDIR, "sincos_check")
+    if verbose:
+        print("building", os.path.relpath(out, ROOT), flush=True)
+    os.makedirs(LIBDIR, exist_ok=True)
+    _run([HIPCC] + FLAGS + [os.path.join(CSRC, "sincos_check.hip"), "-o", out])
+    return out
+
+
 def build_all(verbose=True):
     built = []
     for model_dir, out, native in IN_TREE:
@@ -100,6 +110,7 @@ def build_all(verbose=True):
             print("building", os.path.relpath(out, ROOT), flush=True)
         built.append(build_library(model_dir, out, native))
     built.append(build_pmc_copy(verbose))
+    built.append(build_sincos_check(verbose))
     return built
 
 

@@ -25,7 +25,7 @@ struct RingInv {
   float se, rq2;           // ye = se * rho^2, yi = rq2 * ye
   float kBe, kCe, kBi, kCi;
   float kd;                // inner-disk shift phase per unit U: 2 pi d
-  float kdp;               // the same in units of pi: 2 d
+  float kdr;               // the same in revolutions: d
   float cr, cg;            // ring and Gaussian mixing weights (Zsp folded in)
   int gauss;               // Gaussian present
   float mBe, mCe, mBi, mCi;   // cr * (kBe, kCe, kBi, kCi): the ring term already mixed
@@ -53,7 +53,7 @@ __device__ inline void ring_coefs(double Rex, double rq, double fade, double d, 
   q->kBi = (float)(D_PI * (h + h0 - d * (h - h0) / Rex) * Rin * Rin);
   q->kCi = (float)(-2.0 * D_PI * D_PI * (h - h0) * rq * Rin * Rin * Rin);
   q->kd = (float)(2.0 * D_PI * d);
-  q->kdp = (float)(2.0 * d);
+  q->kdr = (float)d;
 }
 
 // Ring visibility cr (Fex - Fin) at rotated coordinates (U, V).
@@ -82,7 +82,7 @@ __device__ inline float2 ring_vis(const RingInv& q, float U, float V, bool any_l
   const float rei = q.mBi * Bi;
   const float imi = q.mCi * U * Ci;
   float sd, cd;
-  sincospi_fast(q.kdp * U, &sd, &cd);
+  sincosrev(q.kdr * U, &sd, &cd);
   float2 F;
   F.x = fmaf(imi, sd, fmaf(-rei, cd, ree));
   F.y = fmaf(-imi, cd, fmaf(-rei, sd, ime));
@@ -97,7 +97,7 @@ struct Sgra9 {
     float cth, sth;       // rotation by th
     float gaU, gaV;       // Gaussian exponent (log2 units) per U^2, V^2
     float kg;             // Gaussian shift phase per unit U: -2 pi (Ri - d)
-    float kgp;            // the same in units of pi: -2 (Ri - d)
+    float kgr;            // the same in revolutions: -(Ri - d)
   };
 
   // p[i] = parameter i (Zsp, Re, rq, ecc, fade, gax, aq, gq, th).
@@ -109,7 +109,7 @@ struct Sgra9 {
     const double d = ecc * (Rex - Rin);
     ring_coefs(Rex, rq, fade, d, &q.r);
     q.kg = (float)(-2.0 * D_PI * (Rin - d));
-    q.kgp = (float)(-2.0 * (Rin - d));
+    q.kgr = (float)(-(Rin - d));
   }
   // The float64 trigonometric invariants (cos th, sin th).  The step kernel
   // keeps them from one proposal to the next and recomputes them only when
@@ -155,7 +155,7 @@ struct Sgra9 {
     if (q.r.gauss) {
       const float ga = __builtin_amdgcn_exp2f(-fmaf(q.gaU, U * U, q.gaV * (V * V)));
       float sg, cgs;
-      sincospi_fast(q.kgp * U, &sg, &cgs);
+      sincosrev(q.kgr * U, &sg, &cgs);
       const float gw = q.r.cg * ga;
       out.x = fmaf(gw, cgs, out.x);
       out.y = fmaf(gw, sg, out.y);
@@ -172,7 +172,7 @@ struct Sgra13 {
     float cth, sth;       // ring rotation by th
     float cb, sb;         // Gaussian rotation by beta
     float gaU, gaV;
-    float kgu, kgv;       // Gaussian phase per unit (u, v) in units of pi: -2 gr Re (cos alpha, sin alpha)
+    float kgu, kgv;       // Gaussian phase per unit (u, v) in revolutions: -gr Re (cos alpha, sin alpha)
     int gonly;            // gq == 1: Gaussian only
   };
 
@@ -201,7 +201,7 @@ struct Sgra13 {
       const float Vb = fmaf(v, q.cb, -u * q.sb);
       const float ga = __builtin_amdgcn_exp2f(-fmaf(q.gaU, Ub * Ub, q.gaV * (Vb * Vb)));
       float sg, cgs;
-      sincospi_fast(fmaf(q.kgu, u, q.kgv * v), &sg, &cgs);
+      sincosrev(fmaf(q.kgu, u, q.kgv * v), &sg, &cgs);
       const float gw = q.r.cg * ga;
       out.x = fmaf(gw, cgs, out.x);
       out.y = fmaf(gw, sg, out.y);
@@ -240,8 +240,8 @@ __device__ inline Sgra13::Inv Sgra13::prepare(const float* p, const Trig& tg) {
   const double gsy = gax * Rex, gsx = aq * gsy;
   q.gaU = (float)(GCOEF * LOG2E * gsx * gsx);
   q.gaV = (float)(GCOEF * LOG2E * gsy * gsy);
-  q.kgu = (float)(-2.0 * gr * Rex * tg.ca);
-  q.kgv = (float)(-2.0 * gr * Rex * tg.sa);
+  q.kgu = (float)(-gr * Rex * tg.ca);
+  q.kgv = (float)(-gr * Rex * tg.sa);
   q.gonly = (p[8] == 1.0f) ? 1 : 0;
   q.r.gauss = gauss ? 1 : 0;
   q.r.cr = (float)(gauss ? Zsp * (1.0 - gq) : Zsp);

@@ -99,33 +99,6 @@ def odd_even_fit(f_odd, f_even, lim, n_odd, n_even):
     return P, Q
 
 
-def pi_fit(n_odd, n_even):
-    """sin(pi x) = x P(x^2) and cos(pi x) = 1 + x^2 Q(x^2) least-squares fits on [-1/2, 1/2]."""
-    x = 0.5 * np.cos(np.pi * (np.arange(4000) + 0.5) / 4000)
-    x = x[np.abs(x) > 1e-4]
-    s2 = x * x
-    A = np.stack([x * s2 ** k for k in range(n_odd)], 1)
-    P = np.linalg.lstsq(A, np.sin(np.pi * x), rcond=None)[0]
-    A = np.stack([s2 * s2 ** k for k in range(n_even)], 1)
-    Q = np.linalg.lstsq(A, np.cos(np.pi * x) - 1.0, rcond=None)[0]
-    return P, Q
-
-
-def check_pi32(P, Q):
-    x = np.linspace(-0.5, 0.5, 400001).astype(np.float32)
-    x2 = (x * x).astype(np.float32)
-    acc = np.full_like(x2, np.float32(P[-1]))
-    for c in P[-2::-1]:
-        acc = (acc * x2 + np.float32(c)).astype(np.float32)
-    sn = (x * acc).astype(np.float32)
-    acc = np.full_like(x2, np.float32(Q[-1]))
-    for c in Q[-2::-1]:
-        acc = (acc * x2 + np.float32(c)).astype(np.float32)
-    cs = (np.float32(1.0) + x2 * acc).astype(np.float32)
-    xd = x.astype(np.float64)
-    return float(np.max(np.abs(sn - np.sin(np.pi * xd)))), float(np.max(np.abs(cs - np.cos(np.pi * xd))))
-
-
 def check_sincos32(P, Q, lim):
     r = np.linspace(-lim, lim, 200001).astype(np.float32)
     r2 = (r * r).astype(np.float32)
@@ -229,10 +202,6 @@ def main():
     es, ec = check_sincos32(P, Q, np.pi / 2 + 1e-6)
     emit("SIN_T", P, "sin r = r + r^3 sum c_k r^(2k), |r| <= pi/2 (least squares); fp32 max abs err %.2e" % es)
     emit("COS_T", Q, "cos r = 1 + r^2 sum c_k r^(2k), |r| <= pi/2 (least squares); fp32 max abs err %.2e" % ec)
-    P, Q = pi_fit(5, 5)
-    es, ec = check_pi32(P, Q)
-    emit("SINPI_T", P, "sin(pi x) = x sum c_k x^(2k), |x| <= 1/2 (least squares); fp32 max abs err %.2e" % es)
-    emit("COSPI_T", Q, "cos(pi x) = 1 + x^2 sum c_k x^(2k), |x| <= 1/2 (least squares); fp32 max abs err %.2e" % ec)
     ma = atan_fit(10)
     t = np.linspace(0, 1, 20001).astype(np.float32)
     s = (t * t).astype(np.float32)

@@ -117,7 +117,7 @@ struct Fwd9 {
 
 // Forward model of one visibility, keeping what the backward pass needs.
 // The model of Sgra9::vis (same Bessel table, sin/cos of the phases in
-// units of pi) with the full-range Bessel branch; the ring term is mixed
+// revolutions) with the full-range Bessel branch; the ring term is mixed
 // after the fact, as the reverse pass needs the unmixed ring.
 __device__ inline Fwd9 sgra9_fwd(const Sgra9::Inv& q, float u, float v, const BesselTabBCD& bt) {
   Fwd9 f;
@@ -134,12 +134,12 @@ __device__ inline Fwd9 sgra9_fwd(const Sgra9::Inv& q, float u, float v, const Be
   const float ime = q.r.kCe * f.U * f.Ce;
   const float rei = q.r.kBi * f.Bi;
   const float imi = q.r.kCi * f.U * f.Ci;
-  sincospi_fast(q.r.kdp * f.U, &f.sd, &f.cd);
+  sincosrev(q.r.kdr * f.U, &f.sd, &f.cd);
   f.Fx = ree - (rei * f.cd - imi * f.sd);
   f.Fy = ime - (rei * f.sd + imi * f.cd);
   if (q.r.gauss) {
     f.ga = __builtin_amdgcn_exp2f(-fmaf(q.gaU, f.U * f.U, q.gaV * (f.W * f.W)));
-    sincospi_fast(q.kgp * f.U, &f.sg, &f.cgs);
+    sincosrev(q.kgr * f.U, &f.sg, &f.cgs);
     const float gw = q.r.cg * f.ga;
     f.Vx = fmaf(q.r.cr, f.Fx, gw * f.cgs);
     f.Vy = fmaf(q.r.cr, f.Fy, gw * f.sg);

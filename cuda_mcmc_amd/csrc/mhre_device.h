@@ -101,19 +101,17 @@ __device__ inline void sincos_fast(float theta, float* s, float* c) {
   *c = __uint_as_float(__float_as_uint(cs) ^ flip);
 }
 
-// sin(pi t) and cos(pi t): k = rint(t), r = t - k in [-1/2, 1/2] (exact),
-// least-squares kernels in r of degree 9 / 8, sign (-1)^k.  Callers scale
-// their phase constants by 1/pi once per chain, which saves the reduction's
-// multiply and second Cody-Waite step of sincos_fast per evaluation.
-__device__ inline void sincospi_fast(float t, float* s, float* c) {
-  const float k = rintf(t);
-  const float r = t - k;
-  const float r2 = r * r;
-  const float sn = r * horner(SINPI_T, r2);
-  const float cs = fmaf(r2, horner(COSPI_T, r2), 1.0f);
-  const uint32_t flip = ((uint32_t)(int)k & 1u) << 31;
-  *s = __uint_as_float(__float_as_uint(sn) ^ flip);
-  *c = __uint_as_float(__float_as_uint(cs) ^ flip);
+// sin(2 pi h) and cos(2 pi h), h in revolutions: the reduction
+// r = h - rint(h) into [-1/2, 1/2], exact in fp32 (h - floor(h) is not: it
+// rounds 1 + h for small negative h), then the hardware sine and cosine,
+// which take their argument in revolutions (input domain |r| <= 256).
+// Measured on MI355X against float64: max abs error 1.25e-7.  Callers keep
+// their phase constants in revolutions (1/(2 pi) folded in once per chain),
+// so an evaluation costs four instructions.
+__device__ inline void sincosrev(float h, float* s, float* c) {
+  const float r = h - rintf(h);
+  *s = __builtin_amdgcn_sinf(r);
+  *c = __builtin_amdgcn_cosf(r);
 }
 
 // atan2(y, x) in (-pi, pi]: t = min/max in [0,1], atan(t) = t P(t^2), octant fix-up.

@@ -52,29 +52,6 @@ def test_sin_cos_kernels_radians():
     assert np.max(np.abs(cs - np.cos(r.astype(np.float64)))) < 1.5e-7
 
 
-def test_sinpi_cospi_kernels():
-    a = _arrays()
-    x = np.linspace(-0.5, 0.5, 200001).astype(np.float32)
-    x2 = (x * x).astype(np.float32)
-    sn = (x * _horner32(a["SINPI_T"], x2)).astype(np.float32)
-    cs = (np.float32(1) + x2 * _horner32(a["COSPI_T"], x2)).astype(np.float32)
-    xd = x.astype(np.float64)
-    assert np.max(np.abs(sn - np.sin(np.pi * xd))) < 2.5e-7
-    assert np.max(np.abs(cs - np.cos(np.pi * xd))) < 2.5e-7
-    # the reduction t -> (k = rint t, r = t - k) with the sign (-1)^k covers
-    # every phase; spot-check it against sin/cos(pi t) over several periods
-    t = np.linspace(-7.3, 7.3, 100001).astype(np.float32)
-    k = np.rint(t)
-    rr = (t - k).astype(np.float32)
-    rr2 = (rr * rr).astype(np.float32)
-    sgn = np.where(k.astype(np.int64) % 2 == 0, 1.0, -1.0)
-    s = sgn * (rr * _horner32(a["SINPI_T"], rr2)).astype(np.float32)
-    c = sgn * (np.float32(1) + rr2 * _horner32(a["COSPI_T"], rr2)).astype(np.float32)
-    td = t.astype(np.float64)
-    assert np.max(np.abs(s - np.sin(np.pi * td))) < 2.5e-7
-    assert np.max(np.abs(c - np.cos(np.pi * td))) < 2.5e-7
-
-
 def test_bessel_table_piecewise_quadratics():
     a = _arrays()
     tab = a["BESSEL_TAB"].reshape(-1, 8)

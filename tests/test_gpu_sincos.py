@@ -1,0 +1,28 @@
+"""The hardware sine and cosine behind sincosrev (argument in revolutions,
+reduced by h - rint(h)) against float64: the phases of the ring and Gaussian terms
+use them in the fused and gradient kernels, so their error enters every model
+visibility.  The bound 2e-7 keeps the model within the chi^2 tolerance of
+1e-5 relative; the reduction keeps it at any argument size."""
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "cuda_mcmc_amd", "lib", "sincos_check")
+
+pytestmark = pytest.mark.gpu
+
+
+def test_hardware_sincos_accuracy():
+    if not os.path.isfile(EXE):
+        pytest.fail("cuda_mcmc_amd/lib/sincos_check is not built (python -m cuda_mcmc_amd.build)")
+    out = subprocess.run([EXE], capture_output=True, text=True, timeout=120, check=True).stdout
+    print(out)
+    rows = [line.split() for line in out.splitlines() if line.startswith("range")]
+    assert len(rows) == 4
+    for r in rows:
+        es, ec = float(r[3]), float(r[5])
+        # the reduction h - rint(h) is exact, so the error does not grow
+        # with the argument
+        assert es < 2e-7 and ec < 2e-7, r
