@@ -324,7 +324,8 @@ __device__ inline float closure_resid_v(float2 a, float2 b, float2 e, float4 ow)
 }
 
 // Gradient for fits whose closures lie inside the 512-visibility ring window
-// (FitDesc::ring_ok).  One wave per parameter point streams the 64-wide
+// (FitDesc::ring_ok) and whose closure batches split into at most 16
+// collision-free rounds (host side).  One wave per parameter point streams the 64-wide
 // tiles once, forming the closures after every cgroup tiles from an LDS ring
 // of model visibilities, as the step kernel does.  d(arg V)/dV is linear in
 // its scale, so the closure terms of a visibility reduce to one scalar
