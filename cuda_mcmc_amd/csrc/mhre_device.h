@@ -231,10 +231,34 @@ __device__ inline double bcast(double v, int lane) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+// v from another lane of the same 16-lane row, picked by the DPP control
+// CTRL (both 32-bit halves moved by DPP, no LDS round trip).
+template <int CTRL>
+__device__ inline double dpp_double(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// double of one lane, as a wave-uniform value
+__device__ inline double readlane_double(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
+// Sum over the 64 lanes (all active), the same on every lane: inside each
+// 16-lane row by DPP (lane ^ 1, lane ^ 2 by quad permutes, then the half-row
+// and row mirrors, which pair quads and half-rows once every lane of a quad,
+// then of a half-row, holds the same partial sum), then the four row sums.
 __device__ inline double wave_sum(double v) {
-#pragma unroll
-  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
-  return v;
+  v += dpp_double<0xB1>(v);    // quad_perm [1,0,3,2]
+  v += dpp_double<0x4E>(v);    // quad_perm [2,3,0,1]
+  v += dpp_double<0x141>(v);   // row_half_mirror
+  v += dpp_double<0x140>(v);   // row_mirror
+  return (readlane_double(v, 0) + readlane_double(v, 16)) + (readlane_double(v, 32) + readlane_double(v, 48));
 }
 
 __device__ inline void wave_sync_lds() {
