@@ -391,13 +391,11 @@ __global__ void __launch_bounds__(256) k_eht_grad9_ring(GradArgs g) {
     if (j < nvis) {
       const Fwd9 fw = sgra9_fwd(q, d.x, d.y, bt);
       const float m2 = fmaxf(fmaf(fw.Vx, fw.Vx, fw.Vy * fw.Vy), 1e-30f);
+      const float rs = __builtin_amdgcn_rsqf(m2);   // 1/|V|: |V| = m2 rs, 1/|V|^2 = rs^2
       float sa = 0.0f;
-      if (j < nvr) {
-        const float amp = __builtin_amdgcn_sqrtf(m2);
-        sa = -2.0f * (d.z - amp) * d.w * __builtin_amdgcn_rcpf(amp);
-      }
+      if (j < nvr) sa = -2.0f * (d.z - m2 * rs) * d.w * rs;
       // amplitude: s (Vx, Vy); closures: darg(V, S) = S (-Vy, Vx) / |V|^2
-      const float k = rings[j & (RING - 1)] * __builtin_amdgcn_rcpf(m2);
+      const float k = rings[j & (RING - 1)] * (rs * rs);
       sgra9_bwd(q, fw, fmaf(sa, fw.Vx, -k * fw.Vy), fmaf(sa, fw.Vy, k * fw.Vx), af);
     }
   };
