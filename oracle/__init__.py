@@ -16,8 +16,10 @@ models    fp64 ring+Gaussian visibility models (9- and 13-parameter) and the
           toy models (Rosenbrock, solvsys, quadratic systems).  Pinned by
           closed forms (zero spacing, Airy disk, pure Gaussian), Hermitian
           symmetry, linearity in Zsp, a direct image-plane Fourier sum of the
-          slashed ring, and an independent restatement of the reference's
-          second fp64 mirror (``src/obsdatafit.py:1476-1608``).
+          slashed ring, and a point-by-point restatement (in the tests) of
+          the reference's second fp64 mirror ``src/obsdatafit.py:1476-1608``;
+          the quadratic systems by the reference's solvsys roots, a
+          hand-computed case and the antisymmetric-part invariant.
 chi2      closures and chi^2 terms (``model.cuh:73-111``,
           ``gpu_mcmc.cu:305-323``).  Pinned by hand-computed 3-station cases.
 mhre      the MH-RE update rule (``gpu_mcmc.cu:245-626``) driven by the same

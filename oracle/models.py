@@ -23,7 +23,13 @@ sgra_model9   pinned: zero-spacing flux V(0,0)=Zsp, the uniform-disk (Airy)
 sgra_model13  pinned: reduces to sgra_model9 when eta = th and the 13-param
               Gaussian is placed where the 9-param one is; V(0,0)=Zsp;
               Hermitian symmetry.
-rosenbrock, solvsys, quadsys  pinned by their known roots/minima.
+rosenbrock, solvsys  pinned by their known roots/minima.
+quadsys       pinned by its reduction to the reference's solvsys system
+              (Q_0 = I, b_1 = (-0.25, 1)) and that system's closed-form roots,
+              a hand-computed non-symmetric case, the vanishing of the
+              antisymmetric part of Q and a brute-force double sum.
+sgra_model9 is also compared with a point-by-point restatement of the
+reference's second fp64 mirror, src/obsdatafit.py:1476-1608 (tests).
 
 Units follow the reference: u, v in Gigalambda; radii and Gaussian axes in
 microarcseconds (``model.cuh:366``).
