@@ -83,23 +83,12 @@ def build_library(model_dir, out, native_eht=False, tag=None):
     return out
 
 
-def build_pmc_copy(verbose=True):
-    """The HBM byte-counter calibration program (cuda_mcmc_amd/lib/pmc_copy)."""
-    out = os.path.join(LIBDIR, "pmc_copy")
+def build_program(src, out, verbose=True):
+    """A stand-alone HIP program (test and measurement helpers) with the library's flags."""
     if verbose:
         print("building", os.path.relpath(out, ROOT), flush=True)
-    os.makedirs(LIBDIR, exist_ok=True)
-    _run([HIPCC] + FLAGS + [os.path.join(CSRC, "pmc_copy.hip"), "-o", out])
-    return out
-
-
-def build_sincos_check(verbose=True):
-    """The accuracy check of the hardware sine/cosine (cuda_mcmc_amd/lib/sincos_check)."""
-    out = os.path.join(LIBDIR, "sincos_check")
-    if verbose:
-        print("building", os.path.relpath(out, ROOT), flush=True)
-    os.makedirs(LIBDIR, exist_ok=True)
-    _run([HIPCC] + FLAGS + [os.path.join(CSRC, "sincos_check.hip"), "-o", out])
+    os.makedirs(os.path.dirname(out), exist_ok=True)
+    _run([HIPCC] + FLAGS + [src, "-o", out])
     return out
 
 
@@ -109,8 +98,6 @@ def build_all(verbose=True):
         if verbose:
             print("building", os.path.relpath(out, ROOT), flush=True)
         built.append(build_library(model_dir, out, native))
-    built.append(build_pmc_copy(verbose))
-    built.append(build_sincos_check(verbose))
     return built
 
 

@@ -384,6 +384,7 @@ struct Session {
   std::vector<KindWork> kinds;
   MhreOptions opt{};
   int wpb = 4, persistent = 0, grid = 0, ncu = 1;
+  int waves_per_simd = 0, lds_per_cu = 0;   // occupancy cap of the native step launches
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   int graph_iters = 0;
@@ -394,6 +395,7 @@ struct Session {
   float* d_tr_chi2 = nullptr;
   int* d_tr_ex = nullptr;
   int tr_nit = 0, tr_it0 = 0;
+  size_t tr_dec_total = 0, tr_ex_total = 0;
   std::vector<void*> extra;   // other device allocations
 };
 
@@ -570,8 +572,21 @@ StepLaunch make_launch(Session* S, const KindWork& k) {
   L.persistent = S->persistent;
   L.grid = S->grid;
   L.qslot = k.qslot;
+  L.waves_per_simd = S->waves_per_simd;
+  L.lds_per_cu = S->lds_per_cu;
   L.tr_dec = S->d_tr_dec;
   L.tr_chi2 = S->d_tr_chi2;
+  L.tr_ex = S->d_tr_ex;
+  L.tr_it0 = S->tr_it0;
+  L.tr_nit = S->tr_nit;
+  return L;
+}
+
+// The exchange launch of an iteration (with the decision trace, if any).
+StepLaunch exchange_launch(Session* S) {
+  StepLaunch L{};
+  L.fits = S->d_fits;
+  L.clk = S->d_clk;
   L.tr_ex = S->d_tr_ex;
   L.tr_it0 = S->tr_it0;
   L.tr_nit = S->tr_nit;
@@ -588,13 +603,7 @@ int launch_iteration(Session* S) {
     else e = launch_step_user(L, S->stream);
     HIPCHK(e);
   }
-  StepLaunch L{};
-  L.fits = S->d_fits;
-  L.clk = S->d_clk;
-  L.tr_ex = S->d_tr_ex;
-  L.tr_it0 = S->tr_it0;
-  L.tr_nit = S->tr_nit;
-  HIPCHK(launch_exchange(L, S->d_x, S->xr_x, S->stream));
+  HIPCHK(launch_exchange(exchange_launch(S), S->d_x, S->xr_x, S->stream));
   return MHRE_OK;
 }
 
@@ -626,10 +635,7 @@ int time_kernels(Session* S, int niter, double* step_ms, double* exch_ms) {
         HIPCHK(err);
       }
       HIPCHK(hipEventRecord(ev[3 * i + 1], S->stream));
-      StepLaunch L{};
-      L.fits = S->d_fits;
-      L.clk = S->d_clk;
-      HIPCHK(launch_exchange(L, S->d_x, S->xr_x, S->stream));
+      HIPCHK(launch_exchange(exchange_launch(S), S->d_x, S->xr_x, S->stream));
       HIPCHK(hipEventRecord(ev[3 * i + 2], S->stream));
     }
     HIPCHK(hipEventSynchronize(ev[3 * (nb - 1) + 2]));
@@ -720,6 +726,7 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
   if (opt) S->opt = *opt;
   S->wpb = S->opt.waves_per_block == 1 || S->opt.waves_per_block == 2 ? S->opt.waves_per_block : 4;
   S->persistent = S->opt.persistent > 0 ? 1 : 0;
+  S->waves_per_simd = std::max(0, S->opt.waves_per_simd);
 
   const int it0 = specs[0].it0;
   S->fits.resize(nfits);
@@ -804,6 +811,20 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
   clk.xit = it0;
   HIPCHK(hipMemcpyAsync(S->d_clk, &clk, sizeof(clk), hipMemcpyHostToDevice, S->stream));
 
+  // decision-trace layout: each fit's [trace_iters][nprm][nchain] decisions and
+  // [trace_iters][nbeta][nseq] exchanges, concatenated in fit order
+  std::vector<long long> tr_dec_off(nfits, 0), tr_ex_off(nfits, 0);
+  size_t tr_dec_total = 0, tr_ex_total = 0;
+  if (S->opt.trace_iters > 0) {
+    for (int i = 0; i < nfits; ++i) {
+      const FitHost& h = S->fits[i];
+      tr_dec_off[i] = (long long)tr_dec_total;
+      tr_ex_off[i] = (long long)tr_ex_total;
+      tr_dec_total += (size_t)S->opt.trace_iters * h.s.nprm * h.nchain;
+      tr_ex_total += (size_t)S->opt.trace_iters * h.nchain;
+    }
+  }
+
   std::vector<FitDesc> descs(nfits);
   Uploads U;
   for (int i = 0; i < nfits; ++i) {
@@ -843,6 +864,8 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
     d.mc = at<CCalcModel>(S, h.o_mc);
     d.nevals = S->d_nevals + 2 * i;
     d.errflag = S->d_err;
+    d.tr_dec = tr_dec_off[i];
+    d.tr_ex = tr_ex_off[i];
     descs[i] = d;
 
     U.add(h.o_vis, h.vis.data(), h.vis.size());
@@ -980,18 +1003,23 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
   HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
   S->ncu = ncu;
   S->grid = std::max(1, ncu * (16 / S->wpb));
+  int lds = 0;
+  if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess || lds <= 0)
+    lds = 160 << 10;
+  S->lds_per_cu = lds;
 
-  // decision trace (single fit)
-  if (S->opt.trace_iters > 0 && nfits == 1) {
-    const FitHost& h = S->fits[0];
+  // decision trace of iterations it0 .. it0 + trace_iters - 1
+  if (S->opt.trace_iters > 0) {
     S->tr_nit = S->opt.trace_iters;
     S->tr_it0 = it0;
-    const size_t nd = (size_t)S->tr_nit * h.s.nprm * h.nchain;
-    CHK(dev_alloc(S, &S->d_tr_dec, nd));
-    CHK(dev_alloc(S, &S->d_tr_chi2, nd));
-    CHK(dev_alloc(S, &S->d_tr_ex, (size_t)S->tr_nit * h.nchain));
-    HIPCHK(hipMemsetAsync(S->d_tr_dec, 0xff, nd * sizeof(int), S->stream));
-    HIPCHK(hipMemsetAsync(S->d_tr_ex, 0xff, (size_t)S->tr_nit * h.nchain * sizeof(int), S->stream));
+    S->tr_dec_total = tr_dec_total;
+    S->tr_ex_total = tr_ex_total;
+    CHK(dev_alloc(S, &S->d_tr_dec, tr_dec_total));
+    CHK(dev_alloc(S, &S->d_tr_chi2, tr_dec_total));
+    CHK(dev_alloc(S, &S->d_tr_ex, tr_ex_total));
+    HIPCHK(hipMemsetAsync(S->d_tr_dec, 0xff, tr_dec_total * sizeof(int), S->stream));
+    HIPCHK(hipMemsetAsync(S->d_tr_chi2, 0, tr_dec_total * sizeof(float), S->stream));
+    HIPCHK(hipMemsetAsync(S->d_tr_ex, 0xff, tr_ex_total * sizeof(int), S->stream));
   }
 
   if (!eval_only) {
@@ -1038,8 +1066,11 @@ int run(Session* S, int niter) {
       for (int i = 0; i < G && rc == MHRE_OK; ++i) rc = launch_iteration(S);
       hipGraph_t g = nullptr;
       const hipError_t ec = hipStreamEndCapture(S->stream, &g);
-      if (rc != MHRE_OK) return rc;
-      HIPCHK(ec);
+      if (rc != MHRE_OK || ec != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        if (rc != MHRE_OK) return rc;
+        HIPCHK(ec);
+      }
       S->graph = g;
       HIPCHK(hipGraphInstantiate(&S->gexec, g, nullptr, nullptr, 0));
     }
@@ -1111,12 +1142,11 @@ int download(Session* S, int model_data) {
     }
   }
   if (S->d_tr_dec && S->opt.trace_dec) {
-    const FitHost& h = S->fits[0];
-    const size_t nd = (size_t)S->tr_nit * h.s.nprm * h.nchain;
-    HIPCHK(hipMemcpy(S->opt.trace_dec, S->d_tr_dec, nd * sizeof(int), hipMemcpyDeviceToHost));
-    if (S->opt.trace_chi2) HIPCHK(hipMemcpy(S->opt.trace_chi2, S->d_tr_chi2, nd * sizeof(float), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(S->opt.trace_dec, S->d_tr_dec, S->tr_dec_total * sizeof(int), hipMemcpyDeviceToHost));
+    if (S->opt.trace_chi2)
+      HIPCHK(hipMemcpy(S->opt.trace_chi2, S->d_tr_chi2, S->tr_dec_total * sizeof(float), hipMemcpyDeviceToHost));
     if (S->opt.trace_ex)
-      HIPCHK(hipMemcpy(S->opt.trace_ex, S->d_tr_ex, (size_t)S->tr_nit * h.nchain * sizeof(int), hipMemcpyDeviceToHost));
+      HIPCHK(hipMemcpy(S->opt.trace_ex, S->d_tr_ex, S->tr_ex_total * sizeof(int), hipMemcpyDeviceToHost));
   }
   return check_model_error(S);
 }

@@ -60,6 +60,8 @@ struct StepLaunch {
   int persistent;        // 1: grid-stride work queue
   int grid;              // work-groups for the persistent form
   int qslot;             // RunClock::qhead slot of the persistent form
+  int waves_per_simd;    // > 0: cap on resident waves per SIMD (LDS reserved per work-group)
+  int lds_per_cu;        // bytes of LDS per compute unit
   int* tr_dec; float* tr_chi2; int* tr_ex; int tr_it0; int tr_nit;   // optional decision trace
 };
 

@@ -145,10 +145,12 @@ __device__ double eht_chi2_wave(const EhtView& v, const typename M::Inv& q, floa
   int kleft = v.cgroup;
   int2 ti = ldg(v.tinfo, 1);
   float r2cur = __int_as_float(ldg(v.tinfo, 0).y);
-  // tile t needs the large-argument branch when se * r2 > 64, i.e. when r2
-  // exceeds 64 / se: a compare of the (non-negative) float bits, in scalar
-  // registers
-  const uint32_t r2_thr = __builtin_amdgcn_readfirstlane(__float_as_uint(64.0f / q.r.se));
+  // tile t needs the large-argument branch when the larger of the two Bessel
+  // arguments, max(1, rq^2) * se * r2, exceeds 64 (rq > 1 makes the inner
+  // argument the larger one), i.e. when r2 exceeds 64 / (se max(1, rq^2)):
+  // a compare of the (non-negative) float bits, in scalar registers
+  const uint32_t r2_thr =
+      __builtin_amdgcn_readfirstlane(__float_as_uint(64.0f / (q.r.se * fmaxf(1.0f, q.r.rq2))));
   // tile t with its data d; fetches tile t + 1 into dn
   auto tile = [&](const int t, const float4& d, float4& dn) {
     const int i = t * TILE + lane;
@@ -376,7 +378,7 @@ __device__ void step_one_chain(const FitDesc& f, int ch, int it, float2* ring, c
       }
     }
     if (tr.dec && it >= tr.it0 && it < tr.it0 + tr.nit && lane == 0) {
-      const size_t ti = ((size_t)(it - tr.it0) * P + s) * C + ch;
+      const size_t ti = (size_t)f.tr_dec + ((size_t)(it - tr.it0) * P + s) * C + ch;
       tr.dec[ti] = k | ((inprior ? 1 : 0) << 8) | (acc << 9);
       tr.chi2[ti] = inprior ? (float)c2 : 0.0f;
     }
@@ -573,7 +575,7 @@ __device__ unsigned quad_one_chain(const FitDesc& f, int ch, int it, const Trace
       n_cnt[pk] = cnt + 1;
     }
     if (tr.dec && it >= tr.it0 && it < tr.it0 + tr.nit) {
-      const size_t ti = ((size_t)(it - tr.it0) * P + s) * C + ch;
+      const size_t ti = (size_t)f.tr_dec + ((size_t)(it - tr.it0) * P + s) * C + ch;
       gptr(tr.dec)[ti] = k | ((inprior ? 1 : 0) << 8) | (acc << 9);
       gptr(tr.chi2)[ti] = inprior ? (float)c2 : 0.0f;
     }
@@ -650,7 +652,7 @@ __device__ void exchange_wave(const FitDesc& f, int iseq0, int it, bool burn, co
         if (sl == k + 1) { c2 = ck; tc = tk; perm = pk; }
       }
       if (tr.ex && sl == 0 && seq_ok && it >= tr.it0 && it < tr.it0 + tr.nit)
-        tr.ex[((size_t)(it - tr.it0) * B + a) * S + iseq] = k | ((acc ? 1 : 0) << 16);
+        tr.ex[(size_t)f.tr_ex + ((size_t)(it - tr.it0) * B + a) * S + iseq] = k | ((acc ? 1 : 0) << 16);
     }
   }
   // move parameter rows: slot lane receives the row that started in slot perm
@@ -705,7 +707,7 @@ __device__ void exchange_serial(const FitDesc& f, int iseq, int it, bool burn, c
       if (!burn) f.n_exch[k0] += 1;
     }
     if (tr.ex && it >= tr.it0 && it < tr.it0 + tr.nit)
-      tr.ex[((size_t)(it - tr.it0) * B + a) * S + iseq] = k | ((acc ? 1 : 0) << 16);
+      tr.ex[(size_t)f.tr_ex + ((size_t)(it - tr.it0) * B + a) * S + iseq] = k | ((acc ? 1 : 0) << 16);
   }
   if (!burn && f.record) {
     const int itr = it - f.nburn;
@@ -775,21 +777,42 @@ static inline int xcd_grid(const XcdRanges& xr, int per_group) {
   return NXCD * cdiv(mx, per_group);
 }
 
+// Dynamic LDS a step launch reserves (unused by the kernel) so that at most
+// waves_per_simd waves of it are resident per SIMD: a work-group of WPB waves
+// puts one wave on each of WPB SIMDs, so 4 * cap / WPB work-groups may share a
+// compute unit, and each then takes lds_per_cu / that many bytes (in 512-byte
+// allocation granules).  0 when no cap is asked for or the static LDS already
+// enforces it.
+static inline size_t occupancy_lds(const StepLaunch& L, int wpb, size_t static_lds) {
+  if (L.waves_per_simd <= 0 || L.lds_per_cu <= 0) return 0;
+  const int groups = (4 * L.waves_per_simd) / wpb;
+  if (groups <= 0) return 0;
+  const size_t want = ((size_t)L.lds_per_cu / (size_t)groups) / 512 * 512;
+  return want > static_lds ? want - static_lds : 0;
+}
+
+template <class M, int WPB, bool PERSIST>
+static void launch_step_form(const StepLaunch& L, const TraceBuf& tr, hipStream_t s) {
+  constexpr size_t static_lds = sizeof(float2) * WPB * RING + sizeof(float4) * 2 * BESSEL_TAB_N;
+  const size_t dyn = occupancy_lds(L, WPB, static_lds);
+  const int g = PERSIST ? L.grid : xcd_grid(L.xr, WPB);
+  k_step_eht<M, WPB, PERSIST><<<g, 64 * WPB, dyn, s>>>(L.fits, L.work, L.xr, L.clk, tr, L.qslot);
+}
+
 template <class M>
 static void launch_step_m(const StepLaunch& L, hipStream_t s) {
   const TraceBuf tr{L.tr_dec, L.tr_chi2, L.tr_ex, L.tr_it0, L.tr_nit};
   if (L.persistent) {
-    const int g = L.grid;
     switch (L.wpb) {
-      case 1: k_step_eht<M, 1, true><<<g, 64, 0, s>>>(L.fits, L.work, L.xr, L.clk, tr, L.qslot); break;
-      case 2: k_step_eht<M, 2, true><<<g, 128, 0, s>>>(L.fits, L.work, L.xr, L.clk, tr, L.qslot); break;
-      default: k_step_eht<M, 4, true><<<g, 256, 0, s>>>(L.fits, L.work, L.xr, L.clk, tr, L.qslot); break;
+      case 1: launch_step_form<M, 1, true>(L, tr, s); break;
+      case 2: launch_step_form<M, 2, true>(L, tr, s); break;
+      default: launch_step_form<M, 4, true>(L, tr, s); break;
     }
   } else {
     switch (L.wpb) {
-      case 1: k_step_eht<M, 1, false><<<xcd_grid(L.xr, 1), 64, 0, s>>>(L.fits, L.work, L.xr, L.clk, tr, L.qslot); break;
-      case 2: k_step_eht<M, 2, false><<<xcd_grid(L.xr, 2), 128, 0, s>>>(L.fits, L.work, L.xr, L.clk, tr, L.qslot); break;
-      default: k_step_eht<M, 4, false><<<xcd_grid(L.xr, 4), 256, 0, s>>>(L.fits, L.work, L.xr, L.clk, tr, L.qslot); break;
+      case 1: launch_step_form<M, 1, false>(L, tr, s); break;
+      case 2: launch_step_form<M, 2, false>(L, tr, s); break;
+      default: launch_step_form<M, 4, false>(L, tr, s); break;
     }
   }
 }

@@ -51,6 +51,8 @@ struct FitDesc {
   int xlanes;            // exchange: lanes per sequence (a power of two >= nbeta, or 64),
                          // so one wave swaps 64 / xlanes sequences side by side
   long long ndatm;       // length of datm (head nchain*ndat, then the phase tail)
+  long long tr_dec;      // decision trace: this fit's first entry in the session's
+  long long tr_ex;       // dec/chi2 and exchange trace arrays (fits in session order)
 
   // ---- static inputs (device pointers)
   const float4* vis;       // [(ntile + 1) * TILE] (u, v, amp, 1/sigma^2) for the native

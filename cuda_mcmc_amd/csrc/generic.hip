@@ -117,7 +117,7 @@ __device__ unsigned user_one_chain(const FitDesc& f, int ch, int it, const Trace
     told = old;
     tlast = t;
     if (tr.dec && it >= tr.it0 && it < tr.it0 + tr.nit) {
-      const size_t ti = ((size_t)(it - tr.it0) * P + s) * C + ch;
+      const size_t ti = (size_t)f.tr_dec + ((size_t)(it - tr.it0) * P + s) * C + ch;
       tr.dec[ti] = k | ((inprior ? 1 : 0) << 8) | (acc << 9);
       tr.chi2[ti] = inprior ? (float)c2 : 0.0f;
     }

@@ -136,12 +136,14 @@ class Session:
     """Device-resident batch of fits.
 
     options: waves_per_block (1/2/4), persistent (0/1), graph_iters (>0 to
-    replay a captured cycle), trace_iters (single fit: record decisions),
-    force_generic (run imodel 1/2 through the compiled-in model.cuh).
+    replay a captured cycle), trace_iters (record decisions: ``trace`` is a dict
+    of arrays for one fit, a list of such dicts for several),
+    force_generic (run imodel 1/2 through the compiled-in model.cuh),
+    waves_per_simd (cap on the native step kernel's resident waves per SIMD).
     """
 
     def __init__(self, fits, lib=None, waves_per_block=0, persistent=-1, graph_iters=0, trace_iters=0,
-                 force_generic=False):
+                 force_generic=False, waves_per_simd=0):
         self.L = _lib.load(lib)
         self.fits = list(fits)
         self._specs = (MhreFitSpec * len(self.fits))(*[f.spec() for f in self.fits])
@@ -150,19 +152,27 @@ class Session:
         self.opt.persistent = int(persistent)
         self.opt.graph_iters = int(graph_iters)
         self.opt.force_generic = int(bool(force_generic))
-        self.trace = None
+        self.opt.waves_per_simd = int(waves_per_simd)
+        self.trace = None      # single fit: dict of arrays; several fits: one dict per fit
         if trace_iters:
-            if len(self.fits) != 1:
-                raise ValueError("decision traces need a single-fit session")
-            f = self.fits[0]
-            C_ = f.nbeta * f.nseq
-            self.trace = dict(dec=np.full((trace_iters, f.nprm, C_), -1, np.int32),
-                              chi2=np.zeros((trace_iters, f.nprm, C_), np.float32),
-                              ex=np.full((trace_iters, f.nbeta, f.nseq), -1, np.int32))
-            self.opt.trace_iters = int(trace_iters)
-            self.opt.trace_dec = ptr(self.trace["dec"], C.c_int)
-            self.opt.trace_chi2 = ptr(self.trace["chi2"], C.c_float)
-            self.opt.trace_ex = ptr(self.trace["ex"], C.c_int)
+            T = int(trace_iters)
+            nd = [T * f.nprm * f.nbeta * f.nseq for f in self.fits]
+            nx = [T * f.nbeta * f.nseq for f in self.fits]
+            self._tr_dec = np.full(sum(nd), -1, np.int32)
+            self._tr_chi2 = np.zeros(sum(nd), np.float32)
+            self._tr_ex = np.full(sum(nx), -1, np.int32)
+            views, a, b = [], 0, 0
+            for f, n1, n2 in zip(self.fits, nd, nx):
+                views.append(dict(dec=self._tr_dec[a:a + n1].reshape(T, f.nprm, f.nbeta * f.nseq),
+                                  chi2=self._tr_chi2[a:a + n1].reshape(T, f.nprm, f.nbeta * f.nseq),
+                                  ex=self._tr_ex[b:b + n2].reshape(T, f.nbeta, f.nseq)))
+                a += n1
+                b += n2
+            self.trace = views[0] if len(self.fits) == 1 else views
+            self.opt.trace_iters = T
+            self.opt.trace_dec = ptr(self._tr_dec, C.c_int)
+            self.opt.trace_chi2 = ptr(self._tr_chi2, C.c_float)
+            self.opt.trace_ex = ptr(self._tr_ex, C.c_int)
         self.h = C.c_void_p()
         _lib.check(self.L, self.L.mhre_session_create(self._specs, len(self.fits), C.byref(self.opt),
                                                       C.byref(self.h)), "mhre_session_create")

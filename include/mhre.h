@@ -123,13 +123,19 @@ typedef struct MhreOptions {
                                    queue; 0: one wave per chain; -1 = default */
   int graph_iters;              /* >0: capture this many iterations in a
                                    hipGraph and replay it */
-  int trace_iters;              /* single-fit sessions: record the decisions of
-                                   iterations it0 .. it0+trace_iters-1 */
+  int trace_iters;              /* record the decisions of iterations
+                                   it0 .. it0+trace_iters-1; each array below is
+                                   the concatenation, in fit order, of every
+                                   fit's block of the shape given */
   int *trace_dec;               /* [trace_iters][nprm][nbeta*nseq]:
                                    k | inprior<<8 | accept<<9 */
   float *trace_chi2;            /* same shape: proposal chi^2 (0 if out of prior) */
   int *trace_ex;                /* [trace_iters][nbeta][nseq]: k | accept<<16 */
   int force_generic;            /* run imodel 1/2 through the compiled-in model.cuh */
+  int waves_per_simd;           /* native step kernel: at most this many resident
+                                   waves per SIMD (the launch reserves LDS so that
+                                   no more work-groups fit on a compute unit);
+                                   0 = as many as registers and LDS allow */
 } MhreOptions;
 
 typedef struct MhreStats {

@@ -8,7 +8,7 @@
 #include <cstdio>
 #include <vector>
 
-#include "mhre_device.h"
+#include "mhre_device.h"   // cuda_mcmc_amd/csrc (on the include path)
 
 namespace {
 

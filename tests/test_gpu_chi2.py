@@ -61,6 +61,28 @@ def test_sgra9_chi2_matches_oracle(seed):
     np.testing.assert_allclose(chi2m.reshape(C, ndat), terms, rtol=2e-4, atol=1e-6 * terms.max())
 
 
+def test_sgra9_chi2_inner_radius_beyond_outer():
+    """rq in (1, 1.5]: the inner ring's Bessel argument rq^2 ye is the larger one.
+
+    The fused kernels take the table-only Bessel path for a tile only when both
+    arguments stay within the table (y <= 64); with rq > 1 that bound comes from
+    the inner ring.  Baselines up to 8 Glambda and Re up to 30 uas put ye just
+    below 64, so rq^2 ye crosses it on the long-baseline tiles."""
+    d = synthdata.ring_gauss_dataset(nvis=2000, ncph=600, seed=13)
+    rng = np.random.default_rng(13)
+    P = random_params(rng, synthdata.EHT_PMIN, synthdata.EHT_PMAX, 48).reshape(4, 12, 9)
+    P[..., 1] = rng.uniform(27.0, 30.0, size=(4, 12))          # Re: ye close to 64 at 8 Glambda
+    P[..., 2] = rng.uniform(1.25, 1.5, size=(4, 12))           # rq > 1
+    P[0, 0, 2] = 1.05
+    uv2 = d["coor"][0::2] ** 2 + d["coor"][1::2] ** 2
+    x2 = (2 * np.pi * P[..., 1:2].astype(np.float64) * np.pi / 180 / 3600 * 1e3) ** 2 * uv2.max()
+    assert np.all(x2 < 64) and np.sum(x2 * P[..., 2:3].astype(np.float64) ** 2 > 64) >= 40
+    chi2c, _, _ = _calc(d, P, 1)
+    terms, _, _ = eht_chi2_terms(P.reshape(-1, 9), d["coor"], d["dat"], d["std2r"], d["icoor"],
+                                 d["nvis"], d["ncph"], 0, 1)
+    np.testing.assert_allclose(chi2c, terms.sum(-1).reshape(4, 12), rtol=1e-5)
+
+
 def test_sgra9_chi2_at_truth_is_data_sized():
     d = synthdata.ring_gauss_dataset(nvis=2000, ncph=600, seed=3)
     P = np.tile(d["truth"].astype(np.float32), (1, 1, 1))

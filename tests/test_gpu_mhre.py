@@ -36,11 +36,14 @@ def _state_from(pb, fit):
     return st
 
 
-def _compare_iteration(pb, fit, st, tr_mh, tr_ex, gpu_dec, gpu_ex, it, tie_rel=2e-6):
+def _compare_iteration(pb, fit, st, tr_mh, tr_ex, gpu_dec, gpu_ex, it, tie_rel=2e-6, counts=None):
+    """Compares one iteration's decisions; counts (a dict) accumulates the
+    decisions made ("decisions") and those inside the tie band ("ties")."""
     B, S, P = fit.nbeta, fit.nseq, fit.nprm
     C = B * S
     bad_chain = np.zeros(C, bool)
     ntie = 0
+    ndec = 0
     for s in range(P):
         o = tr_mh[s]
         g = gpu_dec[s]
@@ -53,6 +56,7 @@ def _compare_iteration(pb, fit, st, tr_mh, tr_ex, gpu_dec, gpu_ex, it, tie_rel=2
         beta = fit.beta[np.arange(C) // S]
         tie = o["inprior"] & (np.abs(o["margin"]) < tie_tol(beta, o["chi2"], o["chi2"], tie_rel))
         ntie += int(np.sum(tie & live))
+        ndec += int(np.sum(o["inprior"] & live))
         diff = live & (gacc.astype(bool) != o["accept"])
         assert not np.any(diff & ~tie), "accept decision differs away from a tie (iteration %d, substep %d)" % (it, s)
         bad_chain |= tie
@@ -70,42 +74,63 @@ def _compare_iteration(pb, fit, st, tr_mh, tr_ex, gpu_dec, gpu_ex, it, tie_rel=2
             tie = np.abs(o["margin"][a]) < tie_tol(db, c0, c0, tie_rel)
             d = ok & (gacc.astype(bool) != o["accept"][a])
             assert not np.any(d & ~tie), "swap decision differs away from a tie"
+            ntie += int(np.sum(tie & ok))
+            ndec += int(np.sum(ok))
             bad_seq |= tie
+    if counts is not None:
+        counts["decisions"] = counts.get("decisions", 0) + ndec
+        counts["ties"] = counts.get("ties", 0) + ntie
     return bad_seq, ntie
 
 
-def _lockstep(fit, chi2_fn, niters, tie_rel=2e-6, **session_options):
-    pb = _problem(fit, chi2_fn)
-    total_ties = 0
+def _check_state(fit, st, ok, it, tie_rel=2e-6):
+    """The device state of every sequence without a near-tie equals the oracle's."""
+    for name in ("n_cnt", "n_acpt", "n_hist", "tcur", "n_exch", "flag", "ptentn"):
+        g = getattr(fit, name)
+        o = getattr(st, name)
+        assert np.array_equal(g[..., ok], o[..., ok]), "%s differs at iteration %d" % (name, it)
+    np.testing.assert_array_equal(fit.pstp[..., ok], st.pstp[..., ok])
+    np.testing.assert_allclose(fit.pcur[:, ok], st.pcur[:, ok], rtol=1e-6, atol=0)
+    np.testing.assert_allclose(fit.chi2c[:, ok], st.chi2c[:, ok], rtol=max(1e-5, 5 * tie_rel))
+    if it >= fit.nburn and fit.pout is not None:
+        itr = it - fit.nburn
+        np.testing.assert_allclose(fit.pout[:, :, ok, itr], st.pout[:, :, ok, itr], rtol=1e-6)
+        np.testing.assert_array_equal(fit.tout[:, ok, itr], st.tout[:, ok, itr])
+
+
+def _lockstep_batch(fits, chi2_fns, niters, tie_rel=2e-6, counts=None, **session_options):
+    """Lock step of a session of one or more fits: every iteration each fit's
+    oracle restarts from the state the device downloaded.  Returns the number
+    of (fit, sequence) states compared over all iterations."""
+    pbs = [_problem(f, c) for f, c in zip(fits, chi2_fns)]
     compared = 0
-    with Session([fit], trace_iters=niters, **session_options) as ses:
+    with Session(fits, trace_iters=niters, **session_options) as ses:
         ses.download()
+        traces = ses.trace if isinstance(ses.trace, list) else [ses.trace]
         for it in range(niters):
-            st = _state_from(pb, fit)
-            trace = {}
-            burn = it < fit.nburn
-            mhre.iteration(pb, st, it, burn, itr_out=None if burn else it - fit.nburn, trace=trace)
+            states, otr = [], []
+            for fit, pb in zip(fits, pbs):
+                st = _state_from(pb, fit)
+                trace = {}
+                burn = it < fit.nburn
+                mhre.iteration(pb, st, it, burn, itr_out=None if burn else it - fit.nburn, trace=trace)
+                states.append(st)
+                otr.append(trace)
             ses.run(1)
             ses.download()
-            dec = ses.trace["dec"][it]
-            ex = ses.trace["ex"][it]
-            bad_seq, nt = _compare_iteration(pb, fit, st, trace["mh"], trace["ex"], dec, ex, it, tie_rel)
-            total_ties += nt
-            ok = ~bad_seq
-            compared += int(ok.sum())
-            B, S = fit.nbeta, fit.nseq
-            for name in ("n_cnt", "n_acpt", "n_hist", "tcur", "n_exch", "flag", "ptentn"):
-                g = getattr(fit, name)
-                o = getattr(st, name)
-                assert np.array_equal(g[..., ok], o[..., ok]), "%s differs at iteration %d" % (name, it)
-            np.testing.assert_array_equal(fit.pstp[..., ok], st.pstp[..., ok])
-            np.testing.assert_allclose(fit.pcur[:, ok], st.pcur[:, ok], rtol=1e-6, atol=0)
-            np.testing.assert_allclose(fit.chi2c[:, ok], st.chi2c[:, ok], rtol=max(1e-5, 5 * tie_rel))
-            if not burn:
-                itr = it - fit.nburn
-                np.testing.assert_allclose(fit.pout[:, :, ok, itr], st.pout[:, :, ok, itr], rtol=1e-6)
-                np.testing.assert_array_equal(fit.tout[:, ok, itr], st.tout[:, ok, itr])
-    return compared, total_ties
+            for fit, pb, st, trace, tv in zip(fits, pbs, states, otr, traces):
+                bad_seq, _ = _compare_iteration(pb, fit, st, trace["mh"], trace["ex"], tv["dec"][it], tv["ex"][it],
+                                                it, tie_rel, counts)
+                ok = ~bad_seq
+                compared += int(ok.sum())
+                _check_state(fit, st, ok, it, tie_rel)
+    return compared
+
+
+def _lockstep(fit, chi2_fn, niters, tie_rel=2e-6, **session_options):
+    counts = {}
+    compared = _lockstep_batch([fit], [chi2_fn], niters, tie_rel, counts, **session_options)
+    return compared, counts
 
 
 def test_init_bit_exact():
@@ -124,8 +149,9 @@ def test_init_bit_exact():
 def test_lockstep_eht_burnin_and_sampling():
     d = synthdata.ring_gauss_dataset(nvis=500, ncph=150, seed=22)
     fit = eht_fit(d, nbeta=8, nseq=16, nburn=3, niter=3, seed=77)
-    compared, ties = _lockstep(fit, oracle_eht_chi2(d), 6)
+    compared, counts = _lockstep(fit, oracle_eht_chi2(d), 6)
     assert compared >= 6 * 16 * 0.5
+    assert counts["ties"] <= 0.02 * counts["decisions"], counts
 
 
 def test_lockstep_adaptation_threshold_and_short_history():
@@ -144,11 +170,13 @@ def test_lockstep_exchange_ladder_shapes(nbeta, nseq):
     # and longer ladders the serial exchange
     d = synthdata.ring_gauss_dataset(nvis=200, ncph=60, seed=24)
     fit = eht_fit(d, nbeta=nbeta, nseq=nseq, nburn=2, niter=2, seed=31 + nbeta)
-    compared, _ = _lockstep(fit, oracle_eht_chi2(d), 4)
-    # a long ladder makes 9 B decisions per sequence and iteration, so more
-    # sequences fall inside a near-tie band and are left out of the state
-    # comparison; every decision outside those bands was still compared
-    assert compared >= 2
+    compared, counts = _lockstep(fit, oracle_eht_chi2(d), 4)
+    # a long ladder makes (nprm + 1) B decisions per sequence and iteration, so
+    # a sequence is more often left out of the state comparison for one
+    # decision inside a tie band; the decisions themselves were all compared
+    # and the tie band holds few of them (random starts: chi^2 ~ 1e5-1e7)
+    assert counts["ties"] <= 0.01 * counts["decisions"], counts
+    assert compared >= 1
 
 
 def test_lockstep_fixed_parameters():
@@ -178,3 +206,32 @@ def test_lockstep_sgra13():
     fit = eht_fit(d, nbeta=8, nseq=16, nburn=3, niter=3, seed=88, model="sgra13")
     compared, _ = _lockstep(fit, oracle_eht_chi2(d, imodel=2), 6)
     assert compared >= 6 * 16 * 0.5
+
+
+def test_lockstep_near_converged_at_plain_fit_size():
+    """The regime where decisions are marginal, at the plain fit's data size.
+
+    2000 amplitudes + 600 closure phases (32 tiles, so the step kernel's
+    512-visibility LDS ring wraps four times per chi^2), chains started next
+    to the truth (chi^2 ~ ndat) with steps small enough that -beta/2 dchi^2 is
+    of order one: accept decisions are close calls, and the tie band
+    (2e-6 relative chi^2, ~5e-3 nats here) must hold almost none of them."""
+    from oracle import cport
+    d = synthdata.ring_gauss_dataset(nvis=2000, ncph=600, nsta=8, seed=2)
+    nb, ns = 32, 16
+    fit = eht_fit(d, nbeta=nb, nseq=ns, nburn=2, niter=2, seed=5, init_chains=False,
+                  pstp=1e-4 * (synthdata.EHT_PMAX - synthdata.EHT_PMIN))
+    rng = np.random.default_rng(5)
+    t = d["truth"].astype(np.float64)
+    P = np.clip(t * (1 + 1e-3 * rng.standard_normal((nb, ns, 9))), fit.pmint, fit.pmaxt).astype(np.float32)
+    fit.pcur[:] = P
+    fit.chi2c[:] = cport.eht_chi2_batch(P.reshape(-1, 9), d).reshape(nb, ns).astype(np.float32)
+    fit.flag[:] = 1
+    assert np.median(fit.chi2c) < 2 * 2600
+    chi2 = lambda Q: cport.eht_chi2_batch(Q, d)
+    compared, counts = _lockstep(fit, chi2, 4)
+    assert counts["decisions"] > 4 * 9 * nb * ns * 0.9
+    assert counts["ties"] <= 0.01 * counts["decisions"], counts
+    assert compared >= 4 * ns * 0.9
+    # the decisions were close calls: both outcomes occur at beta = 1
+    assert 0 < fit.n_acpt[:, 0].sum() < 4 * 9 * ns

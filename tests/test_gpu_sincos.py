@@ -9,14 +9,14 @@ import subprocess
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-EXE = os.path.join(ROOT, "cuda_mcmc_amd", "lib", "sincos_check")
+EXE = os.path.join(ROOT, "tests", "_build", "sincos_check")
 
 pytestmark = pytest.mark.gpu
 
 
 def test_hardware_sincos_accuracy():
     if not os.path.isfile(EXE):
-        pytest.fail("cuda_mcmc_amd/lib/sincos_check is not built (python -m cuda_mcmc_amd.build)")
+        pytest.fail("tests/_build/sincos_check is not built (python -c 'import __graft_entry__ as g; g.build()')")
     out = subprocess.run([EXE], capture_output=True, text=True, timeout=120, check=True).stdout
     print(out)
     rows = [line.split() for line in out.splitlines() if line.startswith("range")]

@@ -45,3 +45,29 @@ def test_normal_moments():
     w = px.philox4x32(c, 1, 2, 0, 5, 6)
     z = px.normal(w[1], w[2])
     assert abs(z.mean()) < 0.01 and abs(z.var() - 1) < 0.01
+
+
+def test_redraw_fixture_keys_force_the_redraw():
+    """The committed keys make the first word 0xFFFFFFFF at their counters, so
+    the pick equals n and the oracle's redraw loop (gpu_mcmc.cu:369-372,
+    :549-552) takes the next counter's word; the library's host Philox agrees."""
+    import ctypes as C
+    import json
+    import os
+    from cuda_mcmc_amd import _lib
+    from oracle.mhre import _pick
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "redraw_keys.json")))
+    L = _lib.load()
+    for name, n in (("step_pick", 9), ("exchange_pick", 31)):
+        e = fx[name]
+        c0, c1, c2, c3 = e["counter"]
+        w = px.philox4x32(c0, c1, c2, c3, e["key0"], e["key1"])
+        assert int(w[0]) == 0xFFFFFFFF, name
+        out = (C.c_uint * 4)()
+        L.mhre_philox4x32_10((C.c_uint * 4)(c0, c1, c2, c3), (C.c_uint * 2)(e["key0"], e["key1"]), out)
+        assert [int(x) for x in out] == [int(x) for x in w]
+        assert px.pick(w[0], n) == n
+        w1 = px.philox4x32(c0, c1, c2, c3 + 1, e["key0"], e["key1"])[0]
+        k = _pick(n, np.array([w[0]], np.uint64),
+                  lambda r: np.array([px.philox4x32(c0, c1, c2, c3 + r, e["key0"], e["key1"])[0]], np.uint64))
+        assert int(k[0]) == int(px.pick(w1, n)) < n
