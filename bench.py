@@ -12,12 +12,17 @@ value = visibility model evaluations performed per second (G/s), inputs
 resident in HBM, whole job (one GPU).
 
 Also reported:
-  roofline      the fused step kernel's algorithmic FP32 rate (170 flop per
-                visibility-eval, 15 per closure term, 40 per chain setup)
-                over its HIP-event duration, against the 157.3 TFLOP/s peak;
+  roofline      the algorithmic FP32 rate of the timed iterations (170 flop
+                per visibility-eval, 15 per closure term, 40 per chain setup)
+                over their device time (HIP events around the window, which
+                hold the step launches of both sequence groups, overlapping
+                on two streams, and the exchange launches), against the
+                157.3 TFLOP/s peak; also the average duration of one step
+                launch and one exchange launch (HIP events around each);
   cpu_baseline  the float64 C/OpenMP oracle port on the host cores;
-  batched       64 uneven fits (config 3 shapes) in one persistent session,
-                launch form picked by the autotuner, vs the same fits one
+  batched       64 uneven fits (500-20000 visibilities, 16-48 x 64-256
+                chains) in one session, launch form picked by the
+                autotuner, vs the same fits one
                 session at a time (fused native kernel, and the per-datum
                 model.cuh contract path);
   adjoint       chi^2 + gradient of 4096 parameter points against 20000
@@ -77,26 +82,42 @@ def traffic_from_profiles():
     return None
 
 
-def cpu_baseline(d, nthreads):
-    """Time the float64 C/OpenMP oracle port on 2 iterations of the same fit."""
+def headline_options(args):
+    """Launch form of the plain fit's session: 4 waves per work-group, one
+    wave per chain, the 128 sequences in two groups on two streams."""
+    return dict(waves_per_block=4, persistent=0, streams=2)
+
+
+def host_threads():
+    """Threads for the CPU baseline: the process's OpenMP allotment (the GPU
+    box gives each GPU 16 host CPUs and sets OMP_NUM_THREADS to that), else
+    every CPU the process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    return len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+
+
+def cpu_baseline(d, state, it0, iters, nthreads):
+    """The float64 C/OpenMP oracle port run for `iters` iterations of the same
+    fit from the device's state at iteration it0 (sampling stage, recording
+    on), timed on the host."""
     from oracle import cport
     cport.load()
-    fit = eht_fit(d, 32, 128, nburn=5000, niter=0, seed=0)
-    # start from a state drawn like the device's (uniform in the prior)
-    rng = np.random.default_rng(0)
-    pmin, pmax = fit.pmint.astype(np.float64), fit.pmaxt.astype(np.float64)
-    fit.pcur[:] = (pmin + (pmax - pmin) * rng.uniform(size=fit.pcur.shape)).astype(np.float32)
-    fit.chi2c[:] = cport.eht_chi2_batch(fit.pcur.reshape(-1, 9), d, nthreads).reshape(fit.chi2c.shape)
+    fit = eht_fit(d, 32, 128, nburn=it0, niter=iters, seed=0)
+    for k in ("pcur", "chi2c", "pstp", "n_cnt", "n_acpt", "n_hist", "n_exch", "tcur", "flag", "ptent", "ptentn"):
+        getattr(fit, k)[...] = getattr(state, k)
     t = time.time()
     nev = 0
-    iters = 2
-    for it in range(iters):
+    for it in range(it0, it0 + iters):
         nev += cport.eht_iteration(fit, d, it, nthreads)
     dt = time.time() - t
     return dict(value=nev * d["nvis"] / dt / 1e9, unit="G visibility-evals/s", cores=int(nthreads),
-                kind="port",
-                sample="%d MH-RE iterations (all 4096 chains, %d chi^2 evaluations of 2000 vis + 600 "
-                       "closures) of the same fit, float64 C/OpenMP port of the oracle, %.1f s" % (iters, nev, dt))
+                host_cpus=int(os.cpu_count() or 0), kind="port",
+                sample="%d MH-RE iterations (%d..%d, sampling stage, recording on; all 4096 chains, %d chi^2 "
+                       "evaluations of 2000 vis + 600 closures) of the same fit from the device's state, float64 "
+                       "C/OpenMP port of the oracle on %d threads, %.1f s"
+                       % (iters, it0, it0 + iters - 1, nev, nthreads, dt))
 
 
 def batched_vs_plain(iters, nfits, log_every=16):
@@ -120,7 +141,7 @@ def batched_vs_plain(iters, nfits, log_every=16):
     tune_iters = 2
     mk = lambda d, nb, ns, f: eht_fit(d, nb, ns, nburn=iters + 2 + 6 * tune_iters, niter=0, seed=300 + f)
     total_vis = 0.0
-    with Session([mk(d, nb, ns, f) for f, (d, nb, ns) in enumerate(fits)], persistent=1) as s:
+    with Session([mk(d, nb, ns, f) for f, (d, nb, ns) in enumerate(fits)], persistent=0, streams=2) as s:
         s.run(1)
         wpb, pers, _ = s.autotune(tune_iters)
         s.sync()
@@ -131,6 +152,7 @@ def batched_vs_plain(iters, nfits, log_every=16):
         tb = time.time() - t0
         e1 = s.stats()
         total_vis = e1["vis_evals"] - e0["vis_evals"]
+        tb_dev = e1["run_ms"] * 1e-3
     batched = total_vis / tb
     tp = 0.0
     plain_vis = 0.0
@@ -163,7 +185,8 @@ def batched_vs_plain(iters, nfits, log_every=16):
             tg += time.time() - t0
             gen_vis += s.stats()["vis_evals"] - e0["vis_evals"]
     generic = gen_vis / tg
-    return dict(fits=nfits, iterations=iters, autotuned={"waves_per_block": wpb, "persistent": pers},
+    return dict(fits=nfits, iterations=iters, streams=2, autotuned={"waves_per_block": wpb, "persistent": pers},
+                batched_device_gvis_per_s=total_vis / tb_dev / 1e9,
                 plain_generic_sequential_gvis_per_s=generic / 1e9, ratio_vs_generic=batched / generic,
                 chains=int(sum(nb * ns for _, nb, ns in fits)),
                 vis_total=int(sum(d["nvis"] for d, _, _ in fits)),
@@ -273,13 +296,14 @@ def quadsys_leg(nsys=1024, nburn=3000, niter=10000):
 def main():
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=24900)
-    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=25000)
+    ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-fits", type=int, default=64)
-    ap.add_argument("--batched-iters", type=int, default=4)
+    ap.add_argument("--batched-iters", type=int, default=16)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--cpu-iters", type=int, default=8)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-quadsys", action="store_true")
     args = ap.parse_args()
@@ -304,47 +328,63 @@ def main():
     log(mcmc_interf.list_gpus().strip())
     d = synthdata.ring_gauss_dataset(nvis=2000, ncph=600, nsta=8, seed=2)
     nburn_cfg, niter_cfg = 5000, 20000
-    total = args.warmup + args.steps
-    nburn = min(nburn_cfg, total)
-    niter = max(0, min(niter_cfg, total - nburn))
-    fit = eht_fit(d, 32, 128, nburn=nburn, niter=niter, seed=0)
-    C = 32 * 128
-    per_eval = d["nvis"] * F_VIS + d["ncph"] * F_CPH + F_CHAIN
-    with Session([fit]) as ses:
-        # warmup: the first iterations, the last ones with per-launch HIP events
-        ntime = min(args.warmup, 50)
-        if args.warmup - ntime > 0:
-            ses.run(args.warmup - ntime)
+    total_cfg = nburn_cfg + niter_cfg
+    # The timed window: the whole fit when --warmup + --steps cover it (the
+    # default); otherwise --steps iterations of the sampling stage, after the
+    # burn-in and --warmup more untimed iterations, so that a short window
+    # sees adapted step sizes and the sampling-stage recording.
+    if args.warmup + args.steps >= total_cfg:
+        w0 = min(args.warmup, total_cfg)
+    else:
+        w0 = nburn_cfg + args.warmup
+    w1 = min(w0 + args.steps, total_cfg)
+    steps = w1 - w0
+    fit = eht_fit(d, 32, 128, nburn=nburn_cfg, niter=niter_cfg, seed=0)
+    opts = dict(headline_options(args))
+    # sequence groups of the session (the engine caps them at the 64 exchange waves)
+    ngroups = max(1, min(opts.get("streams", 1), 8, 64))
+    with Session([fit], **opts) as ses:
+        if w0 > 0:
+            ses.run(w0)
         ses.sync()
-        e0 = ses.stats()
-        step_ms, exch_ms = ses.time_kernels(ntime) if ntime > 0 else (float("nan"), float("nan"))
         e1 = ses.stats()
-        evals_per_launch = (e1["chi2_evals"] - e0["chi2_evals"]) / max(ntime, 1)
-        log("warmup done: step kernel %.4f ms, exchange %.4f ms per iteration" % (step_ms, exch_ms))
-        # timed region
-        ses.sync()
+        log("untimed iterations 0..%d done" % w0)
+        # timed region: inputs and chain state resident in HBM
         t0 = time.time()
-        ses.run(args.steps)
+        ses.run(steps)
         ses.sync()
         wall = time.time() - t0
         e2 = ses.stats()
         gpu_ms = e2["run_ms"]
-        log("timed %d iterations: %.3f s wall, %.3f s device" % (args.steps, wall, gpu_ms / 1e3))
+        log("timed iterations %d..%d: %.3f s wall, %.3f s device" % (w0, w1 - 1, wall, gpu_ms / 1e3))
         vis = e2["vis_evals"] - e1["vis_evals"]
         cph = e2["cph_evals"] - e1["cph_evals"]
         ev = e2["chi2_evals"] - e1["chi2_evals"]
         props = e2["proposals"] - e1["proposals"]
+    # A second session of the same fit (no recording) run to the first
+    # sampling iteration (or the window's start, if later): its state there
+    # seeds the CPU baseline, and 20 more iterations with HIP events around
+    # every launch give the step and exchange kernels' per-launch durations.
+    p0 = max(w0, nburn_cfg)
+    state = eht_fit(d, 32, 128, nburn=nburn_cfg, niter=niter_cfg, seed=0)
+    state.pout = state.tout = state.chi2 = None
+    nprobe = 20
+    with Session([state], **opts) as ses:
+        ses.run(p0)
+        ses.download()
+        step_ms, exch_ms = ses.time_kernels(nprobe)
+    log("per launch at iterations %d..%d: step %.4f ms, exchange %.4f ms" % (p0, p0 + nprobe - 1, step_ms, exch_ms))
     value = vis / wall / 1e9
-    achieved = evals_per_launch * per_eval / (step_ms * 1e-3) / 1e12
-    flop_rate = (vis * F_VIS + cph * F_CPH + ev * F_CHAIN) / wall / 1e12
+    flop = vis * F_VIS + cph * F_CPH + ev * F_CHAIN
+    achieved = flop / (gpu_ms * 1e-3) / 1e12
     out = {
         "metric": METRIC,
         "value": round(value, 3),
         "unit": "G visibility-evals/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": round(wall * 1e3 / max(args.steps, 1), 5),
+        "ms_per_step": round(wall * 1e3 / max(steps, 1), 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -353,16 +393,24 @@ def main():
         "config": {"workload": "plain MH-RE fit, 9-param ring+Gaussian, 32 temperatures x 128 sequences, "
                                "2000 vis + 600 closures, 5000 burn-in + 20000 sampling iterations",
                    "nbeta": 32, "nseq": 128, "nvis": 2000, "ncph": 600, "nprm": 9,
-                   "nburn": nburn_cfg, "niter": niter_cfg, "philox_seed": 0},
-        "pct_fp32_peak": round(100.0 * flop_rate / PEAK_FP32_TFLOPS, 2),
+                   "nburn": nburn_cfg, "niter": niter_cfg, "philox_seed": 0,
+                   "timed_iterations": [w0, w1], "iterations_run": w1,
+                   "timed_stage": "whole fit" if w0 == 0 and w1 == total_cfg else
+                                  ("sampling (recording on)" if w0 >= nburn_cfg else "burn-in and sampling"),
+                   "launch_form": opts},
+        "pct_fp32_peak": round(100.0 * flop / wall / 1e12 / PEAK_FP32_TFLOPS, 2),
         "proposals_per_s": props / wall,
         "chi2_evals_per_s": ev / wall,
         "roofline": {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                      "traffic": traffic_from_profiles(),
-                     "kernel": "k_step_eht<Sgra9>", "launch_ms": round(step_ms, 5),
-                     "exchange_ms": round(exch_ms, 5),
-                     "flop_per_launch": evals_per_launch * per_eval},
+                     "kernel": "one MH-RE iteration: k_step_eht<Sgra9> on each of %d sequence-group streams "
+                               "(overlapping) + k_exchange" % ngroups,
+                     "iteration_ms": round(gpu_ms / max(steps, 1), 5),
+                     "step_launches_per_iteration": ngroups,
+                     "step_launch_ms": round(step_ms, 5), "exchange_launch_ms": round(exch_ms, 5),
+                     "launch_probe_iterations": [p0, p0 + nprobe],
+                     "flop_per_iteration": flop / max(steps, 1)},
     }
     if not args.no_batched:
         log("batched vs plain (%d fits)..." % args.batched_fits)
@@ -374,9 +422,9 @@ def main():
         log("quadratic systems (1024 x 8 equations)...")
         out["quadsys"] = quadsys_leg()
     if not args.no_cpu_baseline:
-        nth = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        nth = args.cpu_threads or host_threads()
         log("cpu baseline on %d threads..." % nth)
-        out["cpu_baseline"] = cpu_baseline(d, nth)
+        out["cpu_baseline"] = cpu_baseline(d, state, p0, args.cpu_iters, nth)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))

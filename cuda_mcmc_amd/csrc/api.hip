@@ -95,7 +95,9 @@ struct FitHost {
   int quad_n = 0;
   // packed host inputs of the native kernels
   std::vector<float> vis;        // 4 per visibility
+  std::vector<float> rho2;       // u^2 + v^2 per visibility
   std::vector<int> clo;          // 4 per closure
+  std::vector<int> clor;         // 4 per closure: ring byte offsets of the legs, datum
   std::vector<float> clow;       // 4 per closure
   std::vector<int> tinfo;        // 2 per tile: first closure, max u^2+v^2 (float bits)
   std::vector<int> clr;          // gradient: leg rounds per closure (color_closure_legs)
@@ -105,7 +107,7 @@ struct FitHost {
   std::vector<int> sumh;
   std::vector<uint32_t> hbits;   // packed acceptance history
   // arena offsets
-  size_t o_vis, o_clo, o_clow, o_tinfo, o_qcoef;
+  size_t o_vis, o_rho2, o_clo, o_clor, o_clow, o_tinfo, o_qcoef;
   size_t o_coor, o_dat, o_std2r, o_icoor, o_idat, o_pdescr, o_ivar, o_invar, o_pmint, o_pmaxt, o_beta;
   size_t o_pcur, o_chi2c, o_pstp, o_ncnt, o_nacpt, o_nhist, o_nexch, o_sumh, o_tcur, o_flag, o_ptent, o_ptentn;
   size_t o_pout, o_tout, o_chi2, o_datm, o_chi2m, o_mc;
@@ -227,6 +229,7 @@ int pack_eht(FitHost& h, int idx) {
   if (ok) std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return tile_of[x] < tile_of[y]; });
   // one wave of zero padding: lanes prefetch closure c0 + lane
   h.clo.assign(((size_t)ncph + TILE) * 4, 0);
+  h.clor.assign(((size_t)ncph + TILE) * 4, 0);
   h.clow.assign(((size_t)ncph + TILE) * 4, 0.0f);
   for (int q = 0; q < ncph; ++q) {
     const int j = order[q];
@@ -234,6 +237,8 @@ int pack_eht(FitHost& h, int idx) {
     h.clo[4 * q + 1] = tri[3 * j + 1];
     h.clo[4 * q + 2] = tri[3 * j + 2];
     h.clo[4 * q + 3] = nvis + j;   // datum index
+    for (int m = 0; m < 3; ++m) h.clor[4 * q + m] = (tri[3 * j + m] & (RING - 1)) * (int)sizeof(float2);
+    h.clor[4 * q + 3] = nvis + j;
     const double obs = s.dat[nvis + j];
     h.clow[4 * q + 0] = (float)cos(obs);
     h.clow[4 * q + 1] = (float)sin(obs);
@@ -247,12 +252,14 @@ int pack_eht(FitHost& h, int idx) {
       h.tinfo[2 * t] = q;
     }
   }
+  h.rho2.assign((size_t)(h.ntile + 1) * TILE, 0.0f);
+  for (int i = 0; i < h.nvis_k; ++i) {
+    const double u = h.vis[4 * i], v = h.vis[4 * i + 1];
+    h.rho2[i] = (float)(u * u + v * v);
+  }
   for (int t = 0; t < h.ntile; ++t) {
     float mx = 0.0f;
-    for (int i = t * TILE; i < std::min((t + 1) * TILE, h.nvis_k); ++i) {
-      const float u = h.vis[4 * i], v = h.vis[4 * i + 1];
-      mx = std::max(mx, u * u + v * v);
-    }
+    for (int i = t * TILE; i < std::min((t + 1) * TILE, h.nvis_k); ++i) mx = std::max(mx, h.rho2[i]);
     // rounded up so that se * mx bounds every lane's se * (u^2 + v^2)
     mx = std::nextafter(mx * (1.0f + 1e-6f), INFINITY);
     memcpy(&h.tinfo[2 * t + 1], &mx, 4);
@@ -331,6 +338,9 @@ int exchange_lanes(int nbeta) {
 
 int hist_words(int nadj) { return (nadj + 31) / 32; }
 
+// Most concurrent sequence groups (streams) of a session.
+constexpr int MAX_GROUPS = 8;
+
 // n_hist[a][q] (0/1, q = k * nchain + chain) -> bit a % 32 of word [a / 32][q]
 // Packs the 0/1 history n_hist[nadj][npc] into bits and, in the same pass,
 // the running accept sums sum[q] = number of set entries of q.
@@ -367,26 +377,39 @@ struct KindWork {
   int qslot = 0;
 };
 
+// A group of whole sequences (of every fit) whose iterations run on their own
+// stream with their own iteration clock.  Sequences are coupled only through
+// the replica exchange of their own temperature ladder, so groups never share
+// a chain, and the launches of different groups may run side by side: while
+// one group's step launch drains its last waves, another group's waves fill
+// the compute units.  Group 0 runs on the session stream.
+struct Group {
+  hipStream_t stream = nullptr;
+  RunClock* d_clk = nullptr;
+  std::vector<KindWork> kinds;   // step lists of the group's chains
+  WorkItem* d_x = nullptr;       // exchange list of the group's sequences
+  int n_x = 0;
+  XcdRanges xr_x{};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  hipEvent_t done = nullptr;
+};
+
 struct Session {
   hipStream_t stream = nullptr;
   std::vector<FitHost> fits;
   char* arena = nullptr;
   size_t arena_bytes = 0;
   FitDesc* d_fits = nullptr;
-  RunClock* d_clk = nullptr;
   unsigned long long* d_nevals = nullptr;   // [nfits][2]
   int* d_err = nullptr;
   WorkItem* d_init = nullptr;
   int n_init = 0;
-  WorkItem* d_x = nullptr;
-  int n_x = 0;
-  XcdRanges xr_x{};   // exchange list per XCD
-  std::vector<KindWork> kinds;
+  std::vector<KindWork> kinds;   // evaluation lists (every chain) per model kind
+  std::vector<Group> groups;
   MhreOptions opt{};
   int wpb = 4, persistent = 0, grid = 0, ncu = 1;
   int waves_per_simd = 0, lds_per_cu = 0;   // occupancy cap of the native step launches
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
   int graph_iters = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   double last_ms = 0.0;
@@ -422,9 +445,15 @@ int resolve_kind(const MhreFitSpec& s, bool force_generic) {
 
 void destroy_session(Session* S) {
   if (!S) return;
+  for (Group& g : S->groups)
+    if (g.stream) hipStreamSynchronize(g.stream);
   if (S->stream) hipStreamSynchronize(S->stream);
-  if (S->gexec) hipGraphExecDestroy(S->gexec);
-  if (S->graph) hipGraphDestroy(S->graph);
+  for (Group& g : S->groups) {
+    if (g.gexec) hipGraphExecDestroy(g.gexec);
+    if (g.graph) hipGraphDestroy(g.graph);
+    if (g.done) hipEventDestroy(g.done);
+    if (g.stream && g.stream != S->stream) hipStreamDestroy(g.stream);
+  }
   // every device allocation except the arena is owned by S->extra
   for (void* p : S->extra) hipFree(p);
   if (S->arena) hipFree(S->arena);
@@ -561,13 +590,13 @@ struct Downloads {
   }
 };
 
-StepLaunch make_launch(Session* S, const KindWork& k) {
+StepLaunch make_launch(Session* S, const Group& g, const KindWork& k) {
   StepLaunch L{};
   L.fits = S->d_fits;
   L.work = k.d_step;
   L.nwork = (int)k.step.size();
   L.xr = k.xr;
-  L.clk = S->d_clk;
+  L.clk = g.d_clk;
   L.wpb = S->wpb;
   L.persistent = S->persistent;
   L.grid = S->grid;
@@ -582,90 +611,134 @@ StepLaunch make_launch(Session* S, const KindWork& k) {
   return L;
 }
 
-// The exchange launch of an iteration (with the decision trace, if any).
-StepLaunch exchange_launch(Session* S) {
+// The exchange launch of a group's iteration (with the decision trace, if any).
+StepLaunch exchange_launch(Session* S, const Group& g) {
   StepLaunch L{};
   L.fits = S->d_fits;
-  L.clk = S->d_clk;
+  L.clk = g.d_clk;
   L.tr_ex = S->d_tr_ex;
   L.tr_it0 = S->tr_it0;
   L.tr_nit = S->tr_nit;
   return L;
 }
 
-int launch_iteration(Session* S) {
-  for (auto& k : S->kinds) {
-    const StepLaunch L = make_launch(S, k);
+int launch_steps(Session* S, const Group& g) {
+  for (const auto& k : g.kinds) {
+    const StepLaunch L = make_launch(S, g, k);
     hipError_t e = hipSuccess;
-    if (k.kind == K_SGRA9) e = launch_step_eht(MODEL_SGRA9, L, S->stream);
-    else if (k.kind == K_SGRA13) e = launch_step_eht(MODEL_SGRA13, L, S->stream);
-    else if (k.kind == K_QUAD) e = launch_step_quad(k.quad_n, L, S->stream);
-    else e = launch_step_user(L, S->stream);
+    if (k.kind == K_SGRA9) e = launch_step_eht(MODEL_SGRA9, L, g.stream);
+    else if (k.kind == K_SGRA13) e = launch_step_eht(MODEL_SGRA13, L, g.stream);
+    else if (k.kind == K_QUAD) e = launch_step_quad(k.quad_n, L, g.stream);
+    else e = launch_step_user(L, g.stream);
     HIPCHK(e);
   }
-  HIPCHK(launch_exchange(exchange_launch(S), S->d_x, S->xr_x, S->stream));
   return MHRE_OK;
 }
 
-// Iterations with HIP events around every launch (on the session stream):
-// average device time per launch of the step kernels and of the exchange.
-// All launches and events of a batch are queued before the host waits, so
-// the gaps between launches are not counted.
+// One iteration of group g on its stream: the step launches, then the exchange.
+int launch_iteration(Session* S, const Group& g) {
+  CHK(launch_steps(S, g));
+  HIPCHK(launch_exchange(exchange_launch(S, g), g.d_x, g.xr_x, g.stream));
+  return MHRE_OK;
+}
+
+// The other groups' streams start after what the session stream holds so far
+// (hipStreamWaitEvent on ev), and the session stream later waits for their
+// work (join_groups), so one event pair on the session stream brackets all.
+int fork_groups(Session* S, hipEvent_t ev) {
+  for (size_t i = 1; i < S->groups.size(); ++i) HIPCHK(hipStreamWaitEvent(S->groups[i].stream, ev, 0));
+  return MHRE_OK;
+}
+int join_groups(Session* S) {
+  for (size_t i = 1; i < S->groups.size(); ++i) {
+    Group& g = S->groups[i];
+    HIPCHK(hipEventRecord(g.done, g.stream));
+    HIPCHK(hipStreamWaitEvent(S->stream, g.done, 0));
+  }
+  return MHRE_OK;
+}
+
+// Iterations with HIP events around every launch (on each group's stream):
+// average device time per launch of the step kernels and of the exchange,
+// over groups and iterations (launches of different groups overlap).  All
+// launches and events of a batch are queued before the host waits, so the
+// gaps between launches are not counted.
 int time_kernels(Session* S, int niter, double* step_ms, double* exch_ms) {
   (void)hipGetLastError();
-  constexpr int BATCH = 64;
-  std::vector<hipEvent_t> ev(3 * BATCH, nullptr);
+  constexpr int BATCH = 32;
+  const int G = (int)S->groups.size();
+  std::vector<hipEvent_t> ev((size_t)3 * BATCH * G, nullptr);
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   struct Destroy {
     std::vector<hipEvent_t>& v;
     ~Destroy() { for (auto e : v) if (e) hipEventDestroy(e); }
   } destroy{ev};
   double st = 0.0, ex = 0.0;
+  HIPCHK(hipEventRecord(S->ev0, S->stream));
+  CHK(fork_groups(S, S->ev0));
   for (int done = 0; done < niter;) {
     const int nb = std::min(BATCH, niter - done);
     for (int i = 0; i < nb; ++i) {
-      HIPCHK(hipEventRecord(ev[3 * i], S->stream));
-      for (auto& k : S->kinds) {
-        const StepLaunch L = make_launch(S, k);
-        hipError_t err = hipSuccess;
-        if (k.kind == K_SGRA9) err = launch_step_eht(MODEL_SGRA9, L, S->stream);
-        else if (k.kind == K_SGRA13) err = launch_step_eht(MODEL_SGRA13, L, S->stream);
-        else if (k.kind == K_QUAD) err = launch_step_quad(k.quad_n, L, S->stream);
-        else err = launch_step_user(L, S->stream);
-        HIPCHK(err);
+      for (int gi = 0; gi < G; ++gi) {
+        const Group& g = S->groups[gi];
+        hipEvent_t* e = &ev[(size_t)3 * (i * G + gi)];
+        HIPCHK(hipEventRecord(e[0], g.stream));
+        CHK(launch_steps(S, g));
+        HIPCHK(hipEventRecord(e[1], g.stream));
+        HIPCHK(launch_exchange(exchange_launch(S, g), g.d_x, g.xr_x, g.stream));
+        HIPCHK(hipEventRecord(e[2], g.stream));
       }
-      HIPCHK(hipEventRecord(ev[3 * i + 1], S->stream));
-      HIPCHK(launch_exchange(exchange_launch(S), S->d_x, S->xr_x, S->stream));
-      HIPCHK(hipEventRecord(ev[3 * i + 2], S->stream));
     }
-    HIPCHK(hipEventSynchronize(ev[3 * (nb - 1) + 2]));
-    for (int i = 0; i < nb; ++i) {
+    for (int gi = 0; gi < G; ++gi) HIPCHK(hipEventSynchronize(ev[(size_t)3 * ((nb - 1) * G + gi) + 2]));
+    for (int i = 0; i < nb * G; ++i) {
       float a = 0.0f, b = 0.0f;
-      HIPCHK(hipEventElapsedTime(&a, ev[3 * i], ev[3 * i + 1]));
-      HIPCHK(hipEventElapsedTime(&b, ev[3 * i + 1], ev[3 * i + 2]));
+      HIPCHK(hipEventElapsedTime(&a, ev[(size_t)3 * i], ev[(size_t)3 * i + 1]));
+      HIPCHK(hipEventElapsedTime(&b, ev[(size_t)3 * i + 1], ev[(size_t)3 * i + 2]));
       st += a;
       ex += b;
     }
     done += nb;
     S->iters_done += nb;
   }
-  if (step_ms) *step_ms = niter > 0 ? st / niter : 0.0;
-  if (exch_ms) *exch_ms = niter > 0 ? ex / niter : 0.0;
+  CHK(join_groups(S));
+  HIPCHK(hipStreamSynchronize(S->stream));
+  if (step_ms) *step_ms = niter > 0 ? st / ((double)niter * G) : 0.0;
+  if (exch_ms) *exch_ms = niter > 0 ? ex / ((double)niter * G) : 0.0;
   return MHRE_OK;
 }
 
 void drop_graph(Session* S) {
-  if (S->gexec) hipGraphExecDestroy(S->gexec);
-  if (S->graph) hipGraphDestroy(S->graph);
-  S->gexec = nullptr;
-  S->graph = nullptr;
+  for (Group& g : S->groups) {
+    if (g.gexec) hipGraphExecDestroy(g.gexec);
+    if (g.graph) hipGraphDestroy(g.graph);
+    g.gexec = nullptr;
+    g.graph = nullptr;
+  }
+}
+
+// Device time per iteration of niter iterations of every group (events on
+// the session stream around the forked and joined group streams).
+int time_iterations(Session* S, int niter, double* ms) {
+  HIPCHK(hipEventRecord(S->ev0, S->stream));
+  CHK(fork_groups(S, S->ev0));
+  for (int i = 0; i < niter; ++i)
+    for (const Group& g : S->groups) CHK(launch_iteration(S, g));
+  CHK(join_groups(S));
+  HIPCHK(hipEventRecord(S->ev1, S->stream));
+  HIPCHK(hipEventSynchronize(S->ev1));
+  float t = 0.0f;
+  HIPCHK(hipEventElapsedTime(&t, S->ev0, S->ev1));
+  S->iters_done += niter;
+  *ms = niter > 0 ? t / niter : 0.0;
+  return MHRE_OK;
 }
 
 // Picks the launch form of the wave-per-chain step kernels by timing real
 // iterations (they advance the chains like any other iteration): each
 // (waves per work-group, grid or persistent) candidate runs `iters`
-// iterations under HIP events and the fastest stays.  All forms give
-// bit-identical chains.  A captured graph is dropped and re-captured later.
+// iterations of every group and the one with the shortest device time per
+// iteration stays.  All forms give bit-identical chains.  A captured graph is
+// dropped and re-captured later.
 int autotune(Session* S, int iters, int* best_wpb, int* best_persistent, double* best_ms) {
   static const int cand[6][2] = {{4, 0}, {2, 0}, {1, 0}, {4, 1}, {2, 1}, {1, 1}};
   if (iters <= 0) FAIL(MHRE_EINVAL, "autotune: iters must be positive");
@@ -675,10 +748,10 @@ int autotune(Session* S, int iters, int* best_wpb, int* best_persistent, double*
     S->wpb = c[0];
     S->persistent = c[1];
     S->grid = std::max(1, S->ncu * (16 / S->wpb));
-    double st = 0.0, ex = 0.0;
-    CHK(time_kernels(S, iters, &st, &ex));
-    if (st < best) {
-      best = st;
+    double t = 0.0;
+    CHK(time_iterations(S, iters, &t));
+    if (t < best) {
+      best = t;
       bw = c[0];
       bp = c[1];
     }
@@ -747,7 +820,9 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
     const MhreFitSpec& s = h.s;
     const size_t C = (size_t)h.nchain, P = (size_t)s.nprm, N = (size_t)s.nptot;
     h.o_vis = A.reserve<float>(h.vis.size());
+    h.o_rho2 = A.reserve<float>(h.rho2.size());
     h.o_clo = A.reserve<int>(h.clo.size());
+    h.o_clor = A.reserve<int>(h.clor.size());
     h.o_clow = A.reserve<float>(h.clow.size());
     h.o_tinfo = A.reserve<int>(h.tinfo.size());
     h.o_qcoef = A.reserve<float>(h.qcoef.size());
@@ -801,15 +876,10 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
   S->arena_bytes = A.bytes;
   HIPCHK(hipMalloc((void**)&S->arena, A.bytes));
   CHK(dev_alloc(S, &S->d_fits, nfits));
-  CHK(dev_alloc(S, &S->d_clk, 1));
   CHK(dev_alloc(S, &S->d_nevals, 2 * (size_t)nfits));
   CHK(dev_alloc(S, &S->d_err, 1));
   HIPCHK(hipMemsetAsync(S->d_nevals, 0, 2 * sizeof(unsigned long long) * nfits, S->stream));
   HIPCHK(hipMemsetAsync(S->d_err, 0, sizeof(int), S->stream));
-  RunClock clk{};
-  clk.it = it0;
-  clk.xit = it0;
-  HIPCHK(hipMemcpyAsync(S->d_clk, &clk, sizeof(clk), hipMemcpyHostToDevice, S->stream));
 
   // decision-trace layout: each fit's [trace_iters][nprm][nchain] decisions and
   // [trace_iters][nbeta][nseq] exchanges, concatenated in fit order
@@ -846,7 +916,8 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
     d.adapt_target = s.adapt_target > 0.0f ? s.adapt_target : 0.30f;
     d.quad_n = h.quad_n; d.quad_neq = s.ndat;
     d.record = h.record ? 1 : 0;
-    d.vis = at<float4>(S, h.o_vis); d.clo = at<int4>(S, h.o_clo); d.clow = at<float4>(S, h.o_clow);
+    d.vis = at<float4>(S, h.o_vis); d.rho2 = at<float>(S, h.o_rho2);
+    d.clo = at<int4>(S, h.o_clo); d.clor = at<int4>(S, h.o_clor); d.clow = at<float4>(S, h.o_clow);
     d.tinfo = at<int2>(S, h.o_tinfo); d.qcoef = at<float>(S, h.o_qcoef);
     d.coor = at<float>(S, h.o_coor); d.dat = at<float>(S, h.o_dat); d.std2r = at<float>(S, h.o_std2r);
     d.icoor = at<int>(S, h.o_icoor); d.idat = at<int>(S, h.o_idat);
@@ -869,7 +940,9 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
     descs[i] = d;
 
     U.add(h.o_vis, h.vis.data(), h.vis.size());
+    U.add(h.o_rho2, h.rho2.data(), h.rho2.size());
     U.add(h.o_clo, h.clo.data(), h.clo.size());
+    U.add(h.o_clor, h.clor.data(), h.clor.size());
     U.add(h.o_clow, h.clow.data(), h.clow.size());
     U.add(h.o_tinfo, h.tinfo.data(), h.tinfo.size());
     U.add(h.o_qcoef, h.qcoef.data(), h.qcoef.size());
@@ -921,49 +994,60 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
   HIPCHK(hipMemcpyAsync(S->d_fits, descs.data(), sizeof(FitDesc) * nfits, hipMemcpyHostToDevice, S->stream));
 
   // work lists: fits grouped per kind, heaviest fits first
-  std::vector<WorkItem> init, xw;
+  std::vector<WorkItem> init;
   const int kinds_order[4] = {K_SGRA9, K_SGRA13, K_QUAD, K_USER};
-  int qslot = 0;
   // per-chain step cost ~ data terms evaluated per chi^2
   std::vector<double> fit_cost(nfits);
   for (int i = 0; i < nfits; ++i) fit_cost[i] = (double)S->fits[i].nvis_k + S->fits[i].s.ndat + 1.0;
+  std::vector<std::vector<int>> by_kind(4);
   for (int kk : kinds_order) {
-    std::vector<int> idx;
+    std::vector<int>& idx = by_kind[kk];
     for (int i = 0; i < nfits; ++i)
       if (S->fits[i].kind == kk) idx.push_back(i);
-    if (idx.empty()) continue;
     std::stable_sort(idx.begin(), idx.end(), [&](int a, int b) {
       const FitHost &x = S->fits[a], &y = S->fits[b];
       return (long long)x.nvis_k + x.s.ndat > (long long)y.nvis_k + y.s.ndat;
     });
-    if (kk == K_QUAD) {
-      // one kernel per compiled system size
-      for (int n : {2, 4, 8}) {
-        KindWork kw;
-        kw.kind = kk;
-        kw.quad_n = n;
-        for (int i : idx) {
-          if (S->fits[i].quad_n != n) continue;
-          const int C = S->fits[i].nchain;
-          for (int c = 0; c < C; ++c) { kw.step.push_back({i, c}); kw.eval.push_back({i, c}); }
-          while (kw.step.size() % 64) kw.step.push_back({i, -1});
-        }
-        if (!kw.eval.empty()) S->kinds.push_back(std::move(kw));
-      }
-      continue;
-    }
-    KindWork kw;
-    kw.kind = kk;
-    kw.qslot = qslot++;
-    for (int i : idx) {
-      for (int c = 0; c < S->fits[i].nchain; ++c) { kw.step.push_back({i, c}); kw.eval.push_back({i, c}); }
-      // lane-per-chain kernels: every wave holds chains of one fit
-      if (kk == K_USER)
-        while (kw.step.size() % 64) kw.step.push_back({i, -1});
-    }
-    if (kk != K_USER) kw.xr = deal_xcd(kw.step, fit_cost);
-    S->kinds.push_back(std::move(kw));
   }
+  // evaluation lists: every chain, per kind (per system size for K_QUAD)
+  for (int kk : kinds_order) {
+    const std::vector<int>& idx = by_kind[kk];
+    if (idx.empty()) continue;
+    for (int n : (kk == K_QUAD ? std::vector<int>{2, 4, 8} : std::vector<int>{0})) {
+      KindWork kw;
+      kw.kind = kk;
+      kw.quad_n = n;
+      for (int i : idx) {
+        if (kk == K_QUAD && S->fits[i].quad_n != n) continue;
+        for (int c = 0; c < S->fits[i].nchain; ++c) kw.eval.push_back({i, c});
+      }
+      if (!kw.eval.empty()) S->kinds.push_back(std::move(kw));
+    }
+  }
+  // Sequence groups: each fit's sequences are cut into G contiguous ranges
+  // at multiples of its exchange wave's sequence count (64 / xlanes), so no
+  // exchange wave and no chain is shared between groups.
+  // At most as many groups as the largest fit has exchange waves, so every
+  // group holds sequences (its exchange launch advances its clock, and the
+  // download reads group 0's clock).
+  int max_waves = 1;
+  for (const FitHost& h : S->fits) {
+    const int per = 64 / exchange_lanes(h.s.nbeta);
+    max_waves = std::max(max_waves, (h.s.nseq + per - 1) / per);
+  }
+  // Default: two groups in the grid form (one group's waves fill the compute
+  // units that the other's early-finishing waves free, and its exchange runs
+  // under the other's step launch), one in the persistent form (whose grid
+  // already fills the chip and would make a second group wait).
+  const int want = S->opt.streams > 0 ? S->opt.streams : (S->persistent ? 1 : 2);
+  const int G = std::max(1, std::min({want, MAX_GROUPS, max_waves}));
+  auto seq_range = [&](int i, int g, int* s0, int* s1) {
+    const int nseq = S->fits[i].s.nseq;
+    const int per = 64 / exchange_lanes(S->fits[i].s.nbeta);
+    const int nblk = (nseq + per - 1) / per;
+    *s0 = std::min(nseq, (int)((long long)nblk * g / G) * per);
+    *s1 = std::min(nseq, (int)((long long)nblk * (g + 1) / G) * per);
+  };
   std::vector<double> seq_cost(nfits);
   for (int i = 0; i < nfits; ++i) {
     const FitHost& h = S->fits[i];
@@ -971,31 +1055,94 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
     if (h.s.init_chains)
       for (int c = 0; c < h.nchain; ++c) init.push_back({i, c});
   }
-  {
-    // exchange list, heaviest fits first, dealt like the step lists
-    std::vector<int> order(nfits);
-    for (int i = 0; i < nfits; ++i) order[i] = i;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return seq_cost[a] > seq_cost[b]; });
-    // one item per wave: its first sequence (64 / xlanes sequences per wave)
-    for (int i : order) {
-      const int g = 64 / exchange_lanes(S->fits[i].s.nbeta);
-      for (int q = 0; q < S->fits[i].s.nseq; q += g) xw.push_back({i, q});
+  std::vector<int> xorder(nfits);
+  for (int i = 0; i < nfits; ++i) xorder[i] = i;
+  std::stable_sort(xorder.begin(), xorder.end(), [&](int a, int b) { return seq_cost[a] > seq_cost[b]; });
+  S->groups.resize(G);
+  for (int g = 0; g < G; ++g) {
+    Group& grp = S->groups[g];
+    int qslot = 0;
+    // chains of the group's sequences, ibeta-major as in the state arrays
+    auto chains = [&](int i, std::vector<WorkItem>& out) {
+      int s0, s1;
+      seq_range(i, g, &s0, &s1);
+      const FitHost& h = S->fits[i];
+      for (int b = 0; b < h.s.nbeta; ++b)
+        for (int q = s0; q < s1; ++q) out.push_back({i, b * h.s.nseq + q});
+    };
+    for (int kk : kinds_order) {
+      const std::vector<int>& idx = by_kind[kk];
+      if (idx.empty()) continue;
+      if (kk == K_QUAD) {
+        // one kernel per compiled system size
+        for (int n : {2, 4, 8}) {
+          KindWork kw;
+          kw.kind = kk;
+          kw.quad_n = n;
+          for (int i : idx) {
+            if (S->fits[i].quad_n != n) continue;
+            const size_t before = kw.step.size();
+            chains(i, kw.step);
+            if (kw.step.size() > before)
+              while (kw.step.size() % 64) kw.step.push_back({i, -1});
+          }
+          if (!kw.step.empty()) grp.kinds.push_back(std::move(kw));
+        }
+        continue;
+      }
+      KindWork kw;
+      kw.kind = kk;
+      kw.qslot = qslot++;
+      for (int i : idx) {
+        const size_t before = kw.step.size();
+        chains(i, kw.step);
+        // lane-per-chain kernels: every wave holds chains of one fit
+        if (kk == K_USER && kw.step.size() > before)
+          while (kw.step.size() % 64) kw.step.push_back({i, -1});
+      }
+      if (kw.step.empty()) continue;
+      if (kk != K_USER) kw.xr = deal_xcd(kw.step, fit_cost);
+      grp.kinds.push_back(std::move(kw));
     }
-    S->xr_x = deal_xcd(xw, seq_cost);
+    // exchange list, heaviest fits first, dealt like the step lists;
+    // one item per wave: its first sequence (64 / xlanes sequences per wave)
+    std::vector<WorkItem> xw;
+    for (int i : xorder) {
+      const int per = 64 / exchange_lanes(S->fits[i].s.nbeta);
+      int s0, s1;
+      seq_range(i, g, &s0, &s1);
+      for (int q = s0; q < s1; q += per) xw.push_back({i, q});
+    }
+    grp.xr_x = deal_xcd(xw, seq_cost);
+    grp.n_x = (int)xw.size();
+    CHK(dev_alloc(S, &grp.d_x, xw.size()));
+    if (!xw.empty())
+      HIPCHK(hipMemcpyAsync(grp.d_x, xw.data(), xw.size() * sizeof(WorkItem), hipMemcpyHostToDevice, S->stream));
+    for (auto& k : grp.kinds) {
+      CHK(dev_alloc(S, &k.d_step, k.step.size()));
+      HIPCHK(hipMemcpyAsync(k.d_step, k.step.data(), k.step.size() * sizeof(WorkItem), hipMemcpyHostToDevice,
+                            S->stream));
+    }
+    CHK(dev_alloc(S, &grp.d_clk, 1));
+    RunClock clk{};
+    clk.it = it0;
+    clk.xit = it0;
+    HIPCHK(hipMemcpyAsync(grp.d_clk, &clk, sizeof(clk), hipMemcpyHostToDevice, S->stream));
+    if (g == 0) {
+      grp.stream = S->stream;
+    } else {
+      HIPCHK(hipStreamCreateWithFlags(&grp.stream, hipStreamNonBlocking));
+      HIPCHK(hipEventCreateWithFlags(&grp.done, hipEventDisableTiming));
+    }
   }
   for (auto& k : S->kinds) {
-    CHK(dev_alloc(S, &k.d_step, k.step.size()));
     CHK(dev_alloc(S, &k.d_eval, k.eval.size()));
-    HIPCHK(hipMemcpyAsync(k.d_step, k.step.data(), k.step.size() * sizeof(WorkItem), hipMemcpyHostToDevice, S->stream));
     HIPCHK(hipMemcpyAsync(k.d_eval, k.eval.data(), k.eval.size() * sizeof(WorkItem), hipMemcpyHostToDevice, S->stream));
   }
   S->n_init = (int)init.size();
-  S->n_x = (int)xw.size();
   CHK(dev_alloc(S, &S->d_init, init.size()));
-  CHK(dev_alloc(S, &S->d_x, xw.size()));
   if (!init.empty())
     HIPCHK(hipMemcpyAsync(S->d_init, init.data(), init.size() * sizeof(WorkItem), hipMemcpyHostToDevice, S->stream));
-  HIPCHK(hipMemcpyAsync(S->d_x, xw.data(), xw.size() * sizeof(WorkItem), hipMemcpyHostToDevice, S->stream));
 
   // persistent grid: a few work-groups per CU
   int dev = 0, ncu = 0;
@@ -1057,29 +1204,34 @@ int run(Session* S, int niter) {
   if (niter <= 0) return MHRE_OK;
   (void)hipGetLastError();
   HIPCHK(hipEventRecord(S->ev0, S->stream));
+  CHK(fork_groups(S, S->ev0));
   int left = niter;
   const int G = S->opt.graph_iters;
   if (G > 0 && niter >= G) {
-    if (!S->gexec) {
-      HIPCHK(hipStreamBeginCapture(S->stream, hipStreamCaptureModeThreadLocal));
+    // one graph of G iterations per group, captured on the group's stream
+    for (Group& grp : S->groups) {
+      if (grp.gexec) continue;
+      HIPCHK(hipStreamBeginCapture(grp.stream, hipStreamCaptureModeThreadLocal));
       int rc = MHRE_OK;
-      for (int i = 0; i < G && rc == MHRE_OK; ++i) rc = launch_iteration(S);
+      for (int i = 0; i < G && rc == MHRE_OK; ++i) rc = launch_iteration(S, grp);
       hipGraph_t g = nullptr;
-      const hipError_t ec = hipStreamEndCapture(S->stream, &g);
+      const hipError_t ec = hipStreamEndCapture(grp.stream, &g);
       if (rc != MHRE_OK || ec != hipSuccess) {
         if (g) hipGraphDestroy(g);
         if (rc != MHRE_OK) return rc;
         HIPCHK(ec);
       }
-      S->graph = g;
-      HIPCHK(hipGraphInstantiate(&S->gexec, g, nullptr, nullptr, 0));
+      grp.graph = g;
+      HIPCHK(hipGraphInstantiate(&grp.gexec, g, nullptr, nullptr, 0));
     }
     while (left >= G) {
-      HIPCHK(hipGraphLaunch(S->gexec, S->stream));
+      for (Group& grp : S->groups) HIPCHK(hipGraphLaunch(grp.gexec, grp.stream));
       left -= G;
     }
   }
-  for (int i = 0; i < left; ++i) CHK(launch_iteration(S));
+  for (int i = 0; i < left; ++i)
+    for (const Group& grp : S->groups) CHK(launch_iteration(S, grp));
+  CHK(join_groups(S));
   HIPCHK(hipEventRecord(S->ev1, S->stream));
   S->iters_done += niter;
   return MHRE_OK;
@@ -1129,7 +1281,7 @@ int download(Session* S, int model_data) {
   }
   // resume records in rndst: seed (8 B), next iteration (4 B), chain (4 B)
   int it = 0;
-  HIPCHK(hipMemcpyAsync(&it, &S->d_clk->it, sizeof(int), hipMemcpyDeviceToHost, S->stream));
+  HIPCHK(hipMemcpyAsync(&it, &S->groups[0].d_clk->it, sizeof(int), hipMemcpyDeviceToHost, S->stream));
   HIPCHK(hipStreamSynchronize(S->stream));
   for (FitHost& h : S->fits) {
     if (!h.s.rndst) continue;
@@ -1360,6 +1512,54 @@ int eht_data_create(const float* coor, const float* dat, const float* std2r, con
   s.coor = nullptr; s.dat = nullptr; s.std2r = nullptr; s.icoor = nullptr; s.idat = nullptr;
   *out = E;
   E = nullptr;
+  return MHRE_OK;
+}
+
+// ------------------------------------------ quadratic-system Jacobians
+
+// Device layout of one mhre_quad_jacobian call: the host inputs staged in
+// device memory and the outputs, in one block.
+struct QuadJacLayout {
+  size_t nq, nb, nd, nx, nr, nj, rows_sys;
+  size_t oQ, ob, od, os, oX, orr, oJ, bytes;
+  int nsys, neq, n, rows;
+  bool has_sys;
+};
+
+int quad_jac_layout(const float* Q, const float* b, const float* dat, const int* sys, int nsys, int neq, int n,
+                    const float* X, int rows, float* r, float* J, QuadJacLayout* L) {
+  if (nsys <= 0 || neq <= 0 || n <= 0 || n > MAXP || rows < 0) FAIL(MHRE_EINVAL, "mhre_quad_jacobian: bad sizes");
+  if (!Q || !b || !dat || !X || !r || !J) FAIL(MHRE_EINVAL, "mhre_quad_jacobian: null array");
+  if (!sys && rows > nsys) FAIL(MHRE_EINVAL, "mhre_quad_jacobian: rows %d > nsys %d without a row map", rows, nsys);
+  if (sys)
+    for (int i = 0; i < rows; ++i)
+      if (sys[i] < 0 || sys[i] >= nsys) FAIL(MHRE_EINVAL, "mhre_quad_jacobian: sys[%d] = %d out of range", i, sys[i]);
+  L->nsys = nsys; L->neq = neq; L->n = n; L->rows = rows; L->has_sys = sys != nullptr;
+  L->nq = (size_t)nsys * neq * n * n; L->nb = (size_t)nsys * neq * n; L->nd = (size_t)nsys * neq;
+  L->nx = (size_t)rows * n; L->nr = (size_t)rows * neq; L->nj = L->nr * n;
+  Arena A;
+  L->oQ = A.reserve<float>(L->nq); L->ob = A.reserve<float>(L->nb); L->od = A.reserve<float>(L->nd);
+  L->os = A.reserve<int>(sys ? rows : 0); L->oX = A.reserve<float>(L->nx);
+  L->orr = A.reserve<float>(L->nr); L->oJ = A.reserve<float>(L->nj);
+  L->bytes = A.bytes;
+  return MHRE_OK;
+}
+
+// Stages the inputs into dev (L.bytes long), runs the kernel on the null
+// stream and copies r and J back (synchronous).
+int quad_jac_staged(char* dev, const QuadJacLayout& L, const float* Q, const float* b, const float* dat,
+                    const int* sys, const float* X, float* r, float* J) {
+  HIPCHK(hipMemcpy(dev + L.oQ, Q, L.nq * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dev + L.ob, b, L.nb * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dev + L.od, dat, L.nd * sizeof(float), hipMemcpyHostToDevice));
+  if (sys) HIPCHK(hipMemcpy(dev + L.os, sys, (size_t)L.rows * sizeof(int), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dev + L.oX, X, L.nx * sizeof(float), hipMemcpyHostToDevice));
+  HIPCHK(launch_quad_jac(reinterpret_cast<float*>(dev + L.oQ), reinterpret_cast<float*>(dev + L.ob),
+                         reinterpret_cast<float*>(dev + L.od), sys ? reinterpret_cast<int*>(dev + L.os) : nullptr,
+                         reinterpret_cast<float*>(dev + L.oX), L.rows, L.neq, L.n,
+                         reinterpret_cast<float*>(dev + L.orr), reinterpret_cast<float*>(dev + L.oJ), nullptr));
+  HIPCHK(hipMemcpy(r, dev + L.orr, L.nr * sizeof(float), hipMemcpyDeviceToHost));
+  HIPCHK(hipMemcpy(J, dev + L.oJ, L.nj * sizeof(float), hipMemcpyDeviceToHost));
   return MHRE_OK;
 }
 
@@ -1615,37 +1815,16 @@ int mhre_eht_data_info(const void* handle, int* info, int ninfo) {
 int mhre_quad_jacobian(const float* Q, const float* b, const float* dat, const int* sys, int nsys, int neq, int n,
                        const float* X, int rows, float* r, float* J) {
   (void)hipGetLastError();
-  if (nsys <= 0 || neq <= 0 || n <= 0 || n > MAXP || rows < 0) FAIL(MHRE_EINVAL, "mhre_quad_jacobian: bad sizes");
-  if (!Q || !b || !dat || !X || !r || !J) FAIL(MHRE_EINVAL, "mhre_quad_jacobian: null array");
-  if (!sys && rows > nsys) FAIL(MHRE_EINVAL, "mhre_quad_jacobian: rows %d > nsys %d without a row map", rows, nsys);
-  if (sys)
-    for (int i = 0; i < rows; ++i)
-      if (sys[i] < 0 || sys[i] >= nsys) FAIL(MHRE_EINVAL, "mhre_quad_jacobian: sys[%d] = %d out of range", i, sys[i]);
+  QuadJacLayout Lq;
+  CHK(quad_jac_layout(Q, b, dat, sys, nsys, neq, n, X, rows, r, J, &Lq));
   if (rows == 0) return MHRE_OK;
-  const size_t nq = (size_t)nsys * neq * n * n, nb = (size_t)nsys * neq * n, nd = (size_t)nsys * neq;
-  const size_t nx = (size_t)rows * n, nr = (size_t)rows * neq, nj = nr * n;
-  Arena A;
-  const size_t oQ = A.reserve<float>(nq), ob = A.reserve<float>(nb), od = A.reserve<float>(nd);
-  const size_t os = A.reserve<int>(sys ? rows : 0), oX = A.reserve<float>(nx);
-  const size_t orr = A.reserve<float>(nr), oJ = A.reserve<float>(nj);
   char* dev = nullptr;
-  HIPCHK(hipMalloc((void**)&dev, A.bytes));
+  HIPCHK(hipMalloc((void**)&dev, Lq.bytes));
   struct Free {
     char* p;
     ~Free() { hipFree(p); }
   } guard{dev};
-  HIPCHK(hipMemcpy(dev + oQ, Q, nq * sizeof(float), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(dev + ob, b, nb * sizeof(float), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(dev + od, dat, nd * sizeof(float), hipMemcpyHostToDevice));
-  if (sys) HIPCHK(hipMemcpy(dev + os, sys, rows * sizeof(int), hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(dev + oX, X, nx * sizeof(float), hipMemcpyHostToDevice));
-  HIPCHK(launch_quad_jac(reinterpret_cast<float*>(dev + oQ), reinterpret_cast<float*>(dev + ob),
-                         reinterpret_cast<float*>(dev + od), sys ? reinterpret_cast<int*>(dev + os) : nullptr,
-                         reinterpret_cast<float*>(dev + oX), rows, neq, n, reinterpret_cast<float*>(dev + orr),
-                         reinterpret_cast<float*>(dev + oJ), nullptr));
-  HIPCHK(hipMemcpy(r, dev + orr, nr * sizeof(float), hipMemcpyDeviceToHost));
-  HIPCHK(hipMemcpy(J, dev + oJ, nj * sizeof(float), hipMemcpyDeviceToHost));
-  return MHRE_OK;
+  return quad_jac_staged(dev, Lq, Q, b, dat, sys, X, r, J);
 }
 
 }  // extern "C"

@@ -40,31 +40,39 @@ __device__ inline void mix_ring(RingInv* q) {
   q->mCi = q->cr * q->kCi;
 }
 
-// h, h0 and the ring coefficients (model.cuh:366-375) for radii in nrad.
-__device__ inline void ring_coefs(double Rex, double rq, double fade, double d, RingInv* q) {
+// h, h0 and the ring coefficients (model.cuh:366-375) for radii in nrad,
+// given the inner disk's shift as a fraction of the outer radius, e = d / Re
+// (the models know it without a division: e = ecc (1 - rq), times
+// cos(eta - th) for the 13-parameter model).  With r = rq,
+//   h = (2/pi) / (Re^2 [(1 - r^2 (1 + e)) fade + (1 - r^2 (1 - e))]),
+// the reference's expression with Ri = r Re and d = e Re factored out: one
+// float64 division per proposal.
+__device__ inline void ring_coefs(double Rex, double rq, double fade, double e, RingInv* q) {
   const double Rin = rq * Rex;
-  const double h = (2.0 / D_PI) / ((Rex * Rex - Rin * Rin * (1.0 + d / Rex)) * fade
-                                   + (Rex * Rex - Rin * Rin * (1.0 - d / Rex)));
+  const double d = e * Rex;
+  const double r2 = rq * rq;
+  const double h = (2.0 / D_PI) / ((Rex * Rex) * ((1.0 - r2 * (1.0 + e)) * fade + (1.0 - r2 * (1.0 - e))));
   const double h0 = fade * h;
   q->se = (float)((2.0 * D_PI * Rex) * (2.0 * D_PI * Rex));
-  q->rq2 = (float)(rq * rq);
+  q->rq2 = (float)r2;
   q->kBe = (float)(D_PI * (h + h0) * Rex * Rex);
   q->kCe = (float)(-2.0 * D_PI * D_PI * (h - h0) * Rex * Rex * Rex);
-  q->kBi = (float)(D_PI * (h + h0 - d * (h - h0) / Rex) * Rin * Rin);
+  q->kBi = (float)(D_PI * (h + h0 - e * (h - h0)) * Rin * Rin);
   q->kCi = (float)(-2.0 * D_PI * D_PI * (h - h0) * rq * Rin * Rin * Rin);
   q->kd = (float)(2.0 * D_PI * d);
   q->kdr = (float)d;
 }
 
-// Ring visibility cr (Fex - Fin) at rotated coordinates (U, V).
+// Ring visibility cr (Fex - Fin) at rotated coordinate U and squared
+// baseline length rho2 = u^2 + v^2 (the rotation leaves it unchanged, so
+// callers pass it precomputed and V is never formed).
 // any_large == false: the caller guarantees ye <= 64 on every lane, so only
 // the y <= 64 evaluation bc of B, C runs (BesselPoly or BesselTab).
 // bc takes its argument in units of 1/BC::SCALE (the scale is folded into se
 // once per chain, outside the visibility loop).
 template <class BC>
-__device__ inline float2 ring_vis(const RingInv& q, float U, float V, bool any_large, const BC& bc) {
+__device__ inline float2 ring_vis(const RingInv& q, float U, float rho2, bool any_large, const BC& bc) {
   constexpr float S = BC::SCALE;
-  const float rho2 = fmaf(U, U, V * V);
   const float ye = (q.se * S) * rho2;
   const float yi = q.rq2 * ye;
   float Be, Ce, Bi, Ci;
@@ -96,6 +104,7 @@ struct Sgra9 {
     RingInv r;
     float cth, sth;       // rotation by th
     float gaU, gaV;       // Gaussian exponent (log2 units) per U^2, V^2
+    float gaD;            // gaU - gaV: the exponent is gaV rho^2 + gaD U^2
     float kg;             // Gaussian shift phase per unit U: -2 pi (Ri - d)
     float kgr;            // the same in revolutions: -(Ri - d)
   };
@@ -105,11 +114,11 @@ struct Sgra9 {
   __device__ static void ring(Inv& q, const float* p) {   // Re, rq, ecc, fade
     const double Re = p[1], rq = p[2], ecc = p[3], fade = p[4];
     const double Rex = Re * UAS2NRAD;
-    const double Rin = rq * Rex;
-    const double d = ecc * (Rex - Rin);
-    ring_coefs(Rex, rq, fade, d, &q.r);
-    q.kg = (float)(-2.0 * D_PI * (Rin - d));
-    q.kgr = (float)(-(Rin - d));
+    const double e = ecc * (1.0 - rq);   // d / Re, d = ecc (Re - Ri)
+    ring_coefs(Rex, rq, fade, e, &q.r);
+    const double gs = (rq - e) * Rex;     // Ri - d: the Gaussian's shift
+    q.kg = (float)(-2.0 * D_PI * gs);
+    q.kgr = (float)(-gs);
   }
   // The float64 trigonometric invariants (cos th, sin th).  The step kernel
   // keeps them from one proposal to the next and recomputes them only when
@@ -126,6 +135,7 @@ struct Sgra9 {
     const double gsy = (double)p[5] * Rex, gsx = (double)p[6] * gsy;
     q.gaU = (float)(GCOEF * LOG2E * gsx * gsx);
     q.gaV = (float)(GCOEF * LOG2E * gsy * gsy);
+    q.gaD = (float)(GCOEF * LOG2E * (gsx * gsx - gsy * gsy));
   }
   __device__ static void mixing(Inv& q, const float* p) {   // Zsp, gq (and gax, aq: Gaussian present)
     const double Zsp = p[0], gq = p[7];
@@ -147,20 +157,24 @@ struct Sgra9 {
   }
   __device__ static Inv prepare(const float* p) { return prepare(p, trig(p)); }
 
+  // Visibility at (u, v) with rho2 = u^2 + v^2.  The Gaussian exponent
+  // gaU U^2 + gaV V^2 is formed as gaV rho2 + (gaU - gaV) U^2.  Without a
+  // Gaussian (cg = 0) its term is added as zero, so the code has no branch.
+  template <class BC = BesselPoly>
+  __device__ static float2 vis(const Inv& q, float u, float v, float rho2, bool any_large, const BC& bc = BC()) {
+    const float U = fmaf(u, q.cth, v * q.sth);
+    float2 out = ring_vis(q.r, U, rho2, any_large, bc);
+    const float ga = __builtin_amdgcn_exp2f(-fmaf(q.gaD, U * U, q.gaV * rho2));
+    float sg, cgs;
+    sincosrev(q.kgr * U, &sg, &cgs);
+    const float gw = q.r.cg * ga;
+    out.x = fmaf(gw, cgs, out.x);
+    out.y = fmaf(gw, sg, out.y);
+    return out;
+  }
   template <class BC = BesselPoly>
   __device__ static float2 vis(const Inv& q, float u, float v, bool any_large, const BC& bc = BC()) {
-    const float U = fmaf(u, q.cth, v * q.sth);
-    const float V = fmaf(v, q.cth, -u * q.sth);
-    float2 out = ring_vis(q.r, U, V, any_large, bc);
-    if (q.r.gauss) {
-      const float ga = __builtin_amdgcn_exp2f(-fmaf(q.gaU, U * U, q.gaV * (V * V)));
-      float sg, cgs;
-      sincosrev(q.kgr * U, &sg, &cgs);
-      const float gw = q.r.cg * ga;
-      out.x = fmaf(gw, cgs, out.x);
-      out.y = fmaf(gw, sg, out.y);
-    }
-    return out;
+    return vis(q, u, v, fmaf(u, u, v * v), any_large, bc);
   }
 };
 
@@ -172,6 +186,7 @@ struct Sgra13 {
     float cth, sth;       // ring rotation by th
     float cb, sb;         // Gaussian rotation by beta
     float gaU, gaV;
+    float gaD;            // gaU - gaV (exponent gaV rho^2 + gaD Ub^2)
     float kgu, kgv;       // Gaussian phase per unit (u, v) in revolutions: -gr Re (cos alpha, sin alpha)
     int gonly;            // gq == 1: Gaussian only
   };
@@ -188,25 +203,24 @@ struct Sgra13 {
   __device__ static Inv prepare(const float* p, const Trig& tg);
   __device__ static Inv prepare(const float* p) { return prepare(p, trig(p)); }
 
+  // Visibility at (u, v) with rho2 = u^2 + v^2 (see Sgra9::vis).  The ring
+  // is skipped for a Gaussian-only source, as in the reference.
+  template <class BC = BesselPoly>
+  __device__ static float2 vis(const Inv& q, float u, float v, float rho2, bool any_large, const BC& bc = BC()) {
+    float2 out = make_float2(0.0f, 0.0f);
+    if (!q.gonly) out = ring_vis(q.r, fmaf(u, q.cth, v * q.sth), rho2, any_large, bc);
+    const float Ub = fmaf(u, q.cb, v * q.sb);
+    const float ga = __builtin_amdgcn_exp2f(-fmaf(q.gaD, Ub * Ub, q.gaV * rho2));
+    float sg, cgs;
+    sincosrev(fmaf(q.kgu, u, q.kgv * v), &sg, &cgs);
+    const float gw = q.r.cg * ga;
+    out.x = fmaf(gw, cgs, out.x);
+    out.y = fmaf(gw, sg, out.y);
+    return out;
+  }
   template <class BC = BesselPoly>
   __device__ static float2 vis(const Inv& q, float u, float v, bool any_large, const BC& bc = BC()) {
-    float2 out = make_float2(0.0f, 0.0f);
-    if (!q.gonly) {
-      const float U = fmaf(u, q.cth, v * q.sth);
-      const float V = fmaf(v, q.cth, -u * q.sth);
-      out = ring_vis(q.r, U, V, any_large, bc);
-    }
-    if (q.r.gauss) {
-      const float Ub = fmaf(u, q.cb, v * q.sb);
-      const float Vb = fmaf(v, q.cb, -u * q.sb);
-      const float ga = __builtin_amdgcn_exp2f(-fmaf(q.gaU, Ub * Ub, q.gaV * (Vb * Vb)));
-      float sg, cgs;
-      sincosrev(fmaf(q.kgu, u, q.kgv * v), &sg, &cgs);
-      const float gw = q.r.cg * ga;
-      out.x = fmaf(gw, cgs, out.x);
-      out.y = fmaf(gw, sg, out.y);
-    }
-    return out;
+    return vis(q, u, v, fmaf(u, u, v * v), any_large, bc);
   }
 };
 
@@ -229,9 +243,7 @@ __device__ inline Sgra13::Inv Sgra13::prepare(const float* p, const Trig& tg) {
   const double Zsp = p[0], Re = p[1], rq = p[2], ecc = p[3], fade = p[4];
   const double gr = p[5], gax = p[6], aq = p[7], gq = p[8];
   const double Rex = Re * UAS2NRAD;
-  const double Rin = rq * Rex;
-  const double d = ecc * (Rex - Rin) * tg.ceth;
-  ring_coefs(Rex, rq, fade, d, &q.r);
+  ring_coefs(Rex, rq, fade, ecc * (1.0 - rq) * tg.ceth, &q.r);   // d / Re, d = ecc (Re - Ri) cos(eta - th)
   q.cth = tg.cth;
   q.sth = tg.sth;
   q.cb = tg.cb;
@@ -240,6 +252,7 @@ __device__ inline Sgra13::Inv Sgra13::prepare(const float* p, const Trig& tg) {
   const double gsy = gax * Rex, gsx = aq * gsy;
   q.gaU = (float)(GCOEF * LOG2E * gsx * gsx);
   q.gaV = (float)(GCOEF * LOG2E * gsy * gsy);
+  q.gaD = (float)(GCOEF * LOG2E * (gsx * gsx - gsy * gsy));
   q.kgu = (float)(-gr * Rex * tg.ca);
   q.kgv = (float)(-gr * Rex * tg.sa);
   q.gonly = (p[8] == 1.0f) ? 1 : 0;

@@ -84,7 +84,9 @@ __device__ inline T ldg(const CONSTANT T* base, int idx) {
 
 struct EhtView {
   const GLOBAL float4* vis;
+  const GLOBAL float* rho2;
   const GLOBAL int4* clo;
+  const GLOBAL int4* clor;
   const GLOBAL float4* clow;
   const CONSTANT int2* tinfo;   // per-tile entries: read with scalar loads
   GLOBAL float* datm;
@@ -95,7 +97,8 @@ struct EhtView {
 
 __device__ inline EhtView eht_view(const FitDesc& f) {
   EhtView v;
-  v.vis = gptr(f.vis); v.clo = gptr(f.clo); v.clow = gptr(f.clow); v.tinfo = wave_const(f.tinfo);
+  v.vis = gptr(f.vis); v.rho2 = gptr(f.rho2); v.clo = gptr(f.clo); v.clor = gptr(f.clor);
+  v.clow = gptr(f.clow); v.tinfo = wave_const(f.tinfo);
   v.datm = gptr(f.datm); v.chi2m = gptr(f.chi2m); v.ndatm = f.ndatm;
   v.nvis = f.nvis; v.ncph = f.ncph; v.ntile = f.ntile; v.ring_ok = f.ring_ok; v.cgroup = f.cgroup;
   v.ndat = f.ndat; v.nchain = f.nchain;
@@ -123,24 +126,39 @@ __device__ inline float closure_term(float2 a, float2 b, float2 e, float4 ow, fl
   return dd * dd * ow.z;
 }
 
+// Model visibility in the ring at LDS byte offset off.
+__device__ inline float2 ring_at(const float2* ring, int off) {
+  return *(const float2*)((const char*)ring + off);
+}
+
 // chi^2 of one chain whose model invariants are q.  Lanes stream 64
 // visibilities per step (the next tile's data and the pending closures' inputs
 // are fetched before the model is evaluated); model visibilities go to a
 // per-wave LDS ring, from which the closures are formed after every cgroup-th
 // tile, so that one pass over the closures keeps most lanes busy.
-// The fp32 terms are accumulated in fp64 per lane and summed across the wave.
+// The fp32 terms are summed per lane in fp32 over blocks of 8 tiles (at most
+// ~10 terms), each block sum is added to a per-lane fp64 total, and the
+// totals are summed across the wave.
 // OUT: also write datm/chi2m of chain ch in the reference layout.
 template <class M, bool OUT>
 __device__ double eht_chi2_wave(const EhtView& v, const typename M::Inv& q, float2* ring, const BesselTab& bt,
                                 int ch) {
   const int lane = lane_id();
-  const bool clos = v.ncph > 0 && v.ring_ok;
   const bool out = OUT && v.datm != nullptr;
   const int nvr = v.ndat - v.ncph;   // visibilities with an amplitude datum
   double acc = 0.0;
-  // The packed arrays are padded (vis by one tile of zero weight, closures by
-  // one wave, tinfo by two entries), so every load below is in bounds and the
-  // loads can stay in flight across the model evaluation.
+  float part = 0.0f;
+  // The packed arrays are padded (vis and rho2 by one tile of zero weight,
+  // closures by one wave, tinfo by two entries), so every load below is in
+  // bounds and the loads can stay in flight across the model evaluation.
+  // Tile t of vis / rho2 and closure c of clor / clow are read through
+  // buffer resources at the scalar byte offsets 1024 t, 256 t and 16 c.
+  const uint32_t l16 = (uint32_t)lane * 16u, l4 = (uint32_t)lane * 4u;
+  const BufRsrc vb = make_buf(v.vis, (uint32_t)(v.ntile + 1) * TILE * 16u);
+  const BufRsrc rb = make_buf(v.rho2, (uint32_t)(v.ntile + 1) * TILE * 4u);
+  const BufRsrc cb = make_buf(v.clor, (uint32_t)(v.ncph + TILE) * 16u);
+  const BufRsrc wb = make_buf(v.clow, (uint32_t)(v.ncph + TILE) * 16u);
+  float2* const ring_lane = ring + lane;                       // ring slot of tile t: + 64 (t % 8)
   int c0 = 0;           // first closure not yet formed
   int kleft = v.cgroup;
   int2 ti = ldg(v.tinfo, 1);
@@ -151,48 +169,57 @@ __device__ double eht_chi2_wave(const EhtView& v, const typename M::Inv& q, floa
   // a compare of the (non-negative) float bits, in scalar registers
   const uint32_t r2_thr =
       __builtin_amdgcn_readfirstlane(__float_as_uint(64.0f / (q.r.se * fmaxf(1.0f, q.r.rq2))));
-  // tile t with its data d; fetches tile t + 1 into dn
-  auto tile = [&](const int t, const float4& d, float4& dn) {
-    const int i = t * TILE + lane;
-    dn = ldg(v.vis, i + TILE);
+  // tile t with its data (d, r2); fetches tile t + 1 into (dn, r2n)
+  auto tile = [&](const int t, const float4& d, const float r2, float4& dn, float& r2n) {
+    dn = buf_load_f4(vb, l16, (uint32_t)(t + 1) * 1024u);
+    r2n = buf_load_f(rb, l4, (uint32_t)(t + 1) * 256u);
     const int2 tin = ldg(v.tinfo, t + 2);
-    const int cf = c0 + lane;
-    int4 ix = ldg(v.clo, cf);
-    float4 ow = ldg(v.clow, cf);
+    int4 ix = buf_load_i4(cb, l16, (uint32_t)c0 * 16u);
+    float4 ow = buf_load_f4(wb, l16, (uint32_t)c0 * 16u);
     const bool any_large = __float_as_uint(r2cur) > r2_thr;
-    const float2 V = M::vis(q, d.x, d.y, any_large, bt);
+    const float2 V = M::vis(q, d.x, d.y, r2, any_large, bt);
     const float amp = __builtin_amdgcn_sqrtf(fmaf(V.x, V.x, V.y * V.y));
     const float r = d.z - amp;
-    acc += (double)(r * r * d.w);   // zero weight past the amplitude data
-    if (out && i < nvr) {
-      const size_t imd = (size_t)ch * v.ndat + i;
-      v.datm[imd] = amp;
-      v.chi2m[imd] = r * r * d.w;
-      const long long tail = (long long)v.nchain * v.ndat + (long long)ch * nvr + i;
-      if (tail < v.ndatm) v.datm[tail] = atan2_fast(V.y, V.x);
+    part = fmaf(r * d.w, r, part);   // zero weight past the amplitude data
+    if (out) {
+      const int i = t * TILE + lane;
+      if (i < nvr) {
+        const size_t imd = (size_t)ch * v.ndat + i;
+        v.datm[imd] = amp;
+        v.chi2m[imd] = r * r * d.w;
+        const long long tail = (long long)v.nchain * v.ndat + (long long)ch * nvr + i;
+        if (tail < v.ndatm) v.datm[tail] = atan2_fast(V.y, V.x);
+      }
     }
-    if (clos) {
+    {
       // Lanes past nvis write slots of indices RING back, which no pending
-      // closure reads (the host checked every closure's window).
-      ring[i & (RING - 1)] = V;
+      // closure reads (the host checked every closure's window).  Without
+      // ring-ordered closures (none, or ring_ok = 0) every tinfo entry names
+      // closure 0, so no closure pass runs here.
+      ring_lane[(t & (RING / TILE - 1)) * TILE] = V;
       const int c1 = ti.x;   // closures whose largest index is in tiles <= t
       if (--kleft == 0 || t + 1 == v.ntile) {
         kleft = v.cgroup;
         if (c1 > c0) {
           ring_fence();
-          for (int c = cf; c < c1; c += TILE) {
-            if (c != cf) {
-              ix = ldg(v.clo, c);
-              ow = ldg(v.clow, c);
+          for (int c = c0; c < c1; c += TILE) {
+            if (c != c0) {
+              ix = buf_load_i4(cb, l16, (uint32_t)c * 16u);
+              ow = buf_load_f4(wb, l16, (uint32_t)c * 16u);
             }
-            float mc;
-            const float cterm = closure_term(ring[ix.x & (RING - 1)], ring[ix.y & (RING - 1)],
-                                             ring[ix.z & (RING - 1)], ow, out ? &mc : nullptr);
-            acc += (double)cterm;
-            if (out) {
-              const size_t imd = (size_t)ch * v.ndat + ix.w;
-              v.datm[imd] = mc;
-              v.chi2m[imd] = cterm;
+            // lanes past c1 hold closures of later groups, whose legs are
+            // not in the ring yet
+            if (c + lane < c1) {
+              float mc;
+              // the legs' ring byte offsets come packed (index % RING times 8)
+              const float cterm = closure_term(ring_at(ring, ix.x), ring_at(ring, ix.y), ring_at(ring, ix.z), ow,
+                                               out ? &mc : nullptr);
+              part += cterm;
+              if (out) {
+                const size_t imd = (size_t)ch * v.ndat + ix.w;
+                v.datm[imd] = mc;
+                v.chi2m[imd] = cterm;
+              }
             }
           }
           ring_fence();
@@ -203,13 +230,20 @@ __device__ double eht_chi2_wave(const EhtView& v, const typename M::Inv& q, floa
     r2cur = __int_as_float(ti.y);
     ti = tin;
   };
-  // two tiles per trip: the prefetched tile alternates between da and db,
-  // so the prefetch costs no register copies
-  float4 da = ldg(v.vis, lane), db;
+  // two tiles per trip: the prefetched tile alternates between the a and b
+  // registers, so the prefetch costs no register copies; the fp32 block sum
+  // goes into the fp64 total every fourth trip
+  float4 da = buf_load_f4(vb, l16, 0u), db;
+  float ra = buf_load_f(rb, l4, 0u), rbv;
   for (int t = 0; t < v.ntile; t += 2) {
-    tile(t, da, db);
-    if (t + 1 < v.ntile) tile(t + 1, db, da);
+    tile(t, da, ra, db, rbv);
+    if (t + 1 < v.ntile) tile(t + 1, db, rbv, da, ra);
+    if ((t & 6) == 6) {
+      acc += (double)part;
+      part = 0.0f;
+    }
   }
+  acc += (double)part;
   if (v.ncph && !v.ring_ok) {
     // General closure layout: recompute the three model visibilities.
     for (int c = lane; c < v.ncph; c += TILE) {
@@ -535,9 +569,15 @@ __device__ unsigned quad_one_chain(const FitDesc& f, int ch, int it, const Trace
 #pragma unroll
     for (int i = 0; i < NQ; ++i) old = (i == t) ? x[i] : old;
     const float sd = pstp[pk];
-    double pd = (double)old + (double)sd * normal64(r.y, r.z);
-    if (pdescr[t] == 2) pd = pd - 6.283185307179586 * rint(pd / 6.283185307179586);
-    const float pn = (float)pd;
+    // The Gaussian step in fp32 with the hardware log2 and cosine (which takes
+    // revolutions, so cos(2 pi u2) is one instruction; its absolute error is
+    // below 2e-7): z is within about 2e-6 of the float64 Box-Muller value at
+    // |z| ~ 5, so the proposal may differ from the float64 one by ~1e-6 step
+    // sizes.
+    const float z = __builtin_amdgcn_sqrtf(-1.3862943611198906f * __builtin_amdgcn_logf(u01(r.y))) *
+                    __builtin_amdgcn_cosf(u01(r.z));
+    float pn = fmaf(sd, z, old);
+    if (pdescr[t] == 2) pn = wrap_angle(pn);
     const bool inprior = !(pn < pmint[t] || pn > pmaxt[t]);
     int acc = 0;
     double c2 = 0.0;
@@ -547,7 +587,8 @@ __device__ unsigned quad_one_chain(const FitDesc& f, int ch, int it, const Trace
       c2 = QuadSys<NQ>::chi2(qc, neq, x);
       ++nev;
       const double xx = -0.5 * beta * (c2 - (double)chi2c);
-      if (log((double)u01(r.w)) <= xx) {
+      // log u in fp32 (hardware log2): its error, ~1e-7 |log u|, is far inside the tie band
+      if ((double)(0.69314718055994531f * __builtin_amdgcn_logf(u01(r.w))) <= xx) {
         acc = 1;
         chi2c = (float)c2;
         n_acpt[pk] += 1;

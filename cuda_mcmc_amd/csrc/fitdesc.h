@@ -57,8 +57,11 @@ struct FitDesc {
   // ---- static inputs (device pointers)
   const float4* vis;       // [(ntile + 1) * TILE] (u, v, amp, 1/sigma^2) for the native
                            // kernel, zero past nvis
+  const float* rho2;       // [(ntile + 1) * TILE] u^2 + v^2 of each visibility, zero past nvis
   const int4* clo;         // [ncph + TILE] vis indices (i, j, k, datum index) sorted by
                            // ring tile, zero past ncph
+  const int4* clor;        // [ncph + TILE] the same legs as LDS ring byte offsets
+                           // ((index % RING) * 8) and the datum index
   const float4* clow;      // [ncph + TILE] (cos obs, sin obs, 1/sigma^2, 0)
   const int2* tinfo;       // [ntile + 2] per tile t: first closure whose largest index
                            // lies past tile t - 1, and the max u^2+v^2 of tile t's

@@ -216,6 +216,36 @@ __device__ inline const CONSTANT T* wave_const(const T* p) {
 }
 
 
+// ------------------------------------------------------- buffer resources
+
+// A raw buffer over [base, base + bytes) (stride 0, the gfx9 data format
+// word).  A load through it adds a wave-uniform offset held in a scalar
+// register to the lane's own byte offset, so stepping a wave through an array
+// costs scalar adds only; a load past the end returns zero.
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __amdgpu_buffer_rsrc_t BufRsrc;
+__device__ inline BufRsrc make_buf(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ inline float4 buf_load_f4(BufRsrc r, uint32_t lane_off, uint32_t uni_off) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)lane_off, (int)uni_off, 0);
+  return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]), __uint_as_float(v[3]));
+}
+__device__ inline int4 buf_load_i4(BufRsrc r, uint32_t lane_off, uint32_t uni_off) {
+  const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)lane_off, (int)uni_off, 0);
+  return make_int4((int)v[0], (int)v[1], (int)v[2], (int)v[3]);
+}
+__device__ inline float buf_load_f(BufRsrc r, uint32_t lane_off, uint32_t uni_off) {
+  return __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, (int)lane_off, (int)uni_off, 0));
+}
+#else   // the host pass never runs device code
+struct BufRsrc {};
+__device__ inline BufRsrc make_buf(const void*, uint32_t) { return BufRsrc{}; }
+__device__ inline float4 buf_load_f4(BufRsrc, uint32_t, uint32_t) { return float4{}; }
+__device__ inline int4 buf_load_i4(BufRsrc, uint32_t, uint32_t) { return int4{}; }
+__device__ inline float buf_load_f(BufRsrc, uint32_t, uint32_t) { return 0.0f; }
+#endif
+
 // ---------------------------------------------------------------- wave64
 
 __device__ inline int lane_id() { return (int)(threadIdx.x & 63u); }

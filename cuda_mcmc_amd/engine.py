@@ -139,11 +139,13 @@ class Session:
     replay a captured cycle), trace_iters (record decisions: ``trace`` is a dict
     of arrays for one fit, a list of such dicts for several),
     force_generic (run imodel 1/2 through the compiled-in model.cuh),
-    waves_per_simd (cap on the native step kernel's resident waves per SIMD).
+    waves_per_simd (cap on the native step kernel's resident waves per SIMD),
+    streams (iterate groups of sequences on this many overlapping streams;
+    0 = two in the grid form, one in the persistent form).
     """
 
     def __init__(self, fits, lib=None, waves_per_block=0, persistent=-1, graph_iters=0, trace_iters=0,
-                 force_generic=False, waves_per_simd=0):
+                 force_generic=False, waves_per_simd=0, streams=0):
         self.L = _lib.load(lib)
         self.fits = list(fits)
         self._specs = (MhreFitSpec * len(self.fits))(*[f.spec() for f in self.fits])
@@ -153,6 +155,7 @@ class Session:
         self.opt.graph_iters = int(graph_iters)
         self.opt.force_generic = int(bool(force_generic))
         self.opt.waves_per_simd = int(waves_per_simd)
+        self.opt.streams = int(streams)
         self.trace = None      # single fit: dict of arrays; several fits: one dict per fit
         if trace_iters:
             T = int(trace_iters)

@@ -136,6 +136,13 @@ typedef struct MhreOptions {
                                    waves per SIMD (the launch reserves LDS so that
                                    no more work-groups fit on a compute unit);
                                    0 = as many as registers and LDS allow */
+  int streams;                  /* split every fit's sequences into this many
+                                   groups (1..8, at most the largest fit's
+                                   exchange-wave count), each iterated on its
+                                   own stream so that the groups' launches
+                                   overlap; 0 = default (2 in the grid form, 1
+                                   in the persistent form).  Results do not
+                                   depend on it. */
 } MhreOptions;
 
 typedef struct MhreStats {

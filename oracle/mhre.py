@@ -157,7 +157,8 @@ def mh_substep(pb, st, it, sub, burn, trace=None):
     old = pc[np.arange(C), t].copy()
     st.ptent.reshape(C)[:] = old
     st.ptentn.reshape(C)[:] = t
-    p = old.astype(np.float64) + pstp[k, np.arange(C)].astype(np.float64) * px.normal(w1, w2)
+    sd = pstp[k, np.arange(C)].astype(np.float64)
+    p = old.astype(np.float64) + sd * px.normal(w1, w2)
     ang = pb.pdescr[t] == 2
     p = np.where(ang, wrap_angle(p), p)
     pn = p.astype(st.fd)
@@ -186,7 +187,7 @@ def mh_substep(pb, st, it, sub, burn, trace=None):
         adjust_steps(pb, st, k)
     if trace is not None:
         trace.append(dict(k=k, inprior=inprior, chi2=chi2, accept=accept,
-                          margin=margin))
+                          margin=margin, p=pn, sd=sd, lo=pb.pmint[t], hi=pb.pmaxt[t]))
 
 
 def adjust_steps(pb, st, k):

@@ -209,3 +209,38 @@ def test_gradient_kernel_selection_and_fallback():
     err = np.linalg.norm(g_b - g_a, axis=1) / np.linalg.norm(g_a, axis=1)
     print("shuffled vs time-ordered max row difference", err.max())
     assert np.all(err < 1e-5)
+
+
+@pytest.mark.parametrize("ring", [True, False])
+def test_grad_vs_fd_and_autograd_at_full_size(monkeypatch, ring):
+    """The adjoint-gradient workload's own size: 20000 visibilities + 6000
+    closure phases, where the kernels sum the invariants' adjoints in fp32 over
+    each closure group before adding them to the fp64 totals.  Six points from
+    the prior interior and two next to the truth (where the gradient is small
+    against its individual terms); both gradient kernels, each within 1e-4 of
+    the gradient norm of float64 central differences of the oracle chi^2 (the
+    C port) and of torch autograd."""
+    if not ring:
+        monkeypatch.setenv("MHRE_GRAD_RING", "0")
+    d = _data(nvis=20000, ncph=6000, seed=4)
+    rng = np.random.default_rng(41)
+    span = synthdata.EHT_PMAX - synthdata.EHT_PMIN
+    near = d["truth"].astype(np.float64) + 0.01 * span * rng.standard_normal((2, 9))
+    near = np.clip(near, synthdata.EHT_PMIN + 0.02 * span, synthdata.EHT_PMAX - 0.02 * span)
+    P = np.concatenate([_inner_prior(rng, 6), near.astype(np.float32)])
+    data, chi, g = _gpu_grad(d, P)
+    assert data.info()["ring_gradient"]   # the data qualify; MHRE_GRAD_RING=0 selects the other kernel
+    f = oracle_eht_chi2(d)   # float64 parameters: the FD steps must not be rounded to float32
+    # near the truth each amplitude term (dat - |V|)^2 w carries the fp32
+    # error of |V| (~1e-7 relative, more where the inner disk cancels much of
+    # the outer one) magnified by the signal-to-noise ratio; summed over 26000
+    # terms that is ~1e-5 of chi^2
+    np.testing.assert_allclose(chi, f(P.astype(np.float64)), rtol=2e-5)
+    gfd = fd_grad(f, P)
+    Pt = torch.as_tensor(P.astype(np.float64)).requires_grad_(True)
+    torch_ref.eht_chi2(Pt, d).sum().backward()
+    gat = Pt.grad.numpy()
+    for ref, name in ((gfd, "central differences"), (gat, "torch autograd")):
+        err = np.linalg.norm(g - ref, axis=1) / np.linalg.norm(ref, axis=1)
+        print(name, "row errors", err)
+        assert np.all(err < 1e-4), (name, err)

@@ -8,6 +8,7 @@ change a single bit:
 * hipGraph replay of captured iteration cycles vs eager launches;
 * several fits of different sizes in one session vs each fit alone;
 * the no-recording mode (pout/tout/chi2 NULL) leaves the final state unchanged;
+* sequence groups iterated on separate streams vs one stream;
 * the autotuner, which times every launch form on real iterations, leaves
   the chains as a plain run does.
 
@@ -137,3 +138,21 @@ def test_resume_in_a_second_session(split):
         s.run(f.nburn + f.niter - split)
         s.download()
     _same(ref, f)
+
+
+@pytest.mark.parametrize("streams", [2, 3, 8])
+def test_sequence_groups_on_streams_bit_identical(streams):
+    # sequences cut into groups that iterate on their own streams (with their
+    # own iteration clocks) give the chains of one stream: mixed kinds, one
+    # fit whose sequence count is not a multiple of the group count, a fit
+    # with fewer exchange waves than groups, graph replay and the persistent form
+    def make():
+        d = synthdata.ring_gauss_dataset(nvis=400, ncph=120, seed=97)
+        return [_fit(700, 200, 96, nb=8, ns=40, nburn=10, niter=10),
+                eht_fit(d, 5, 13, nburn=10, niter=10, seed=98, model="sgra13"),
+                _quad(4, 99, nb=6, ns=20)]
+    alone = _run(make())
+    for opt in (dict(persistent=0), dict(persistent=1, graph_iters=4)):
+        got = _run(make(), streams=streams, **opt)
+        for a, b in zip(alone, got):
+            _same(a, b)
