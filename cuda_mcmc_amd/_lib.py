@@ -162,6 +162,8 @@ def _declare(L):
     L.mhre_eht_data_info.argtypes = [vp, C.POINTER(C.c_int), C.c_int]
     L.mhre_quad_jacobian.restype = C.c_int
     L.mhre_quad_jacobian.argtypes = [fp, fp, fp, ip, C.c_int, C.c_int, C.c_int, fp, C.c_int, fp, fp]
+    L.mhre_quad_jacobian_noalloc.restype = C.c_int
+    L.mhre_quad_jacobian_noalloc.argtypes = [fp, fp, fp, ip, C.c_int, C.c_int, C.c_int, fp, C.c_int, fp, fp]
 
 
 class EngineError(RuntimeError):

@@ -18,6 +18,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -1563,6 +1564,35 @@ int quad_jac_staged(char* dev, const QuadJacLayout& L, const float* Q, const flo
   return MHRE_OK;
 }
 
+// ------------------------------------------------ static scratch area
+
+// The staging area of the entry points that allocate nothing: a file-scope
+// device array, one copy on every GPU the library runs on, held until the
+// process ends.  Host code never uses the array's name as an address (on the
+// host it is a stand-in); it uses the device address that hipGetSymbolAddress
+// returns for the current device, looked up once per device.  Calls that
+// overlap on one GPU share the array and race.
+constexpr size_t kScratchBytes = (size_t)32 << 20;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char* g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_on_current_device(char** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices) FAIL(MHRE_EINVAL, "device %d beyond the %d devices the scratch table holds", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = static_cast<char*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return MHRE_OK;
+}
+
 int eht_check_batch(const EhtData* E, const float* P, int n, int ldp, const double* chi2) {
   if (!E) FAIL(MHRE_EINVAL, "null data handle");
   if (n < 0) FAIL(MHRE_EINVAL, "n = %d is negative", n);
@@ -1824,6 +1854,20 @@ int mhre_quad_jacobian(const float* Q, const float* b, const float* dat, const i
     char* p;
     ~Free() { hipFree(p); }
   } guard{dev};
+  return quad_jac_staged(dev, Lq, Q, b, dat, sys, X, r, J);
+}
+
+int mhre_quad_jacobian_noalloc(const float* Q, const float* b, const float* dat, const int* sys, int nsys, int neq,
+                               int n, const float* X, int rows, float* r, float* J) {
+  (void)hipGetLastError();
+  QuadJacLayout Lq;
+  CHK(quad_jac_layout(Q, b, dat, sys, nsys, neq, n, X, rows, r, J, &Lq));
+  if (rows == 0) return MHRE_OK;
+  if (Lq.bytes > kScratchBytes)
+    FAIL(MHRE_EINVAL, "mhre_quad_jacobian_noalloc: the call needs %zu bytes of staging, more than the %zu-byte "
+                      "static scratch area (use mhre_quad_jacobian)", Lq.bytes, kScratchBytes);
+  char* dev = nullptr;
+  CHK(scratch_on_current_device(&dev));
   return quad_jac_staged(dev, Lq, Q, b, dat, sys, X, r, J);
 }
 

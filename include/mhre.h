@@ -234,6 +234,17 @@ int mhre_eht_data_info(const void *handle, int *info, int ninfo);
 int mhre_quad_jacobian(const float *Q, const float *b, const float *dat, const int *sys,
                        int nsys, int neq, int n, const float *X, int rows, float *r, float *J);
 
+/* mhre_quad_jacobian without any device allocation, for callers that cannot
+ * allocate (code under stream capture, tight polishing loops): the inputs are
+ * staged in a static 32 MiB device area of the library (one per GPU, kept
+ * until the process ends) instead of a per-call allocation.  Same arguments,
+ * results and errors, plus MHRE_EINVAL when the call's staging (all of Q, b,
+ * dat, sys, X, r and J, each rounded up to 256 bytes) exceeds 32 MiB.  Calls
+ * that overlap on one GPU share the area and race.  No counterpart in the
+ * reference. */
+int mhre_quad_jacobian_noalloc(const float *Q, const float *b, const float *dat, const int *sys,
+                               int nsys, int neq, int n, const float *X, int rows, float *r, float *J);
+
 #ifdef __cplusplus
 }
 #endif
