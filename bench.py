@@ -306,6 +306,8 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=8)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-quadsys", action="store_true")
+    ap.add_argument("--launch-probe", type=int, default=20,
+                    help="iterations of the per-launch timing probe (0: no probe session)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: every fit and sequence is independent, so the job "
@@ -366,14 +368,18 @@ def main():
     # seeds the CPU baseline, and 20 more iterations with HIP events around
     # every launch give the step and exchange kernels' per-launch durations.
     p0 = max(w0, nburn_cfg)
-    state = eht_fit(d, 32, 128, nburn=nburn_cfg, niter=niter_cfg, seed=0)
-    state.pout = state.tout = state.chi2 = None
-    nprobe = 20
-    with Session([state], **opts) as ses:
-        ses.run(p0)
-        ses.download()
-        step_ms, exch_ms = ses.time_kernels(nprobe)
-    log("per launch at iterations %d..%d: step %.4f ms, exchange %.4f ms" % (p0, p0 + nprobe - 1, step_ms, exch_ms))
+    nprobe = max(0, args.launch_probe)
+    step_ms = exch_ms = None
+    if nprobe or not args.no_cpu_baseline:
+        state = eht_fit(d, 32, 128, nburn=nburn_cfg, niter=niter_cfg, seed=0)
+        state.pout = state.tout = state.chi2 = None
+        with Session([state], **opts) as ses:
+            ses.run(p0)
+            ses.download()
+            if nprobe:
+                step_ms, exch_ms = ses.time_kernels(nprobe)
+                log("per launch at iterations %d..%d: step %.4f ms, exchange %.4f ms"
+                    % (p0, p0 + nprobe - 1, step_ms, exch_ms))
     value = vis / wall / 1e9
     flop = vis * F_VIS + cph * F_CPH + ev * F_CHAIN
     achieved = flop / (gpu_ms * 1e-3) / 1e12
@@ -408,8 +414,9 @@ def main():
                                "(overlapping) + k_exchange" % ngroups,
                      "iteration_ms": round(gpu_ms / max(steps, 1), 5),
                      "step_launches_per_iteration": ngroups,
-                     "step_launch_ms": round(step_ms, 5), "exchange_launch_ms": round(exch_ms, 5),
-                     "launch_probe_iterations": [p0, p0 + nprobe],
+                     "step_launch_ms": None if step_ms is None else round(step_ms, 5),
+                     "exchange_launch_ms": None if exch_ms is None else round(exch_ms, 5),
+                     "launch_probe_iterations": [p0, p0 + nprobe] if nprobe else None,
                      "flop_per_iteration": flop / max(steps, 1)},
     }
     if not args.no_batched:

@@ -339,6 +339,10 @@ int exchange_lanes(int nbeta) {
 
 int hist_words(int nadj) { return (nadj + 31) / 32; }
 
+// Quadratic-system fits whose ladder fits one wave run step and exchange in
+// one launch (k_iter_quad).
+bool quad_fused(const FitHost& h) { return h.kind == K_QUAD && h.s.nbeta <= 64; }
+
 // Most concurrent sequence groups (streams) of a session.
 constexpr int MAX_GROUPS = 8;
 
@@ -371,6 +375,7 @@ void unpack_hist(const std::vector<uint32_t>& bits, int nadj, size_t npc, int* n
 struct KindWork {
   int kind;
   int quad_n;
+  bool fused = false;   // quadratic systems: step + exchange launch, items (fit, first sequence)
   std::vector<WorkItem> step, eval;
   XcdRanges xr{};   // wave-per-chain kinds: step list dealt into per-XCD ranges
   WorkItem* d_step = nullptr;
@@ -624,11 +629,12 @@ StepLaunch exchange_launch(Session* S, const Group& g) {
 }
 
 int launch_steps(Session* S, const Group& g) {
-  for (const auto& k : g.kinds) {
+  for (const KindWork& k : g.kinds) {
     const StepLaunch L = make_launch(S, g, k);
     hipError_t e = hipSuccess;
     if (k.kind == K_SGRA9) e = launch_step_eht(MODEL_SGRA9, L, g.stream);
     else if (k.kind == K_SGRA13) e = launch_step_eht(MODEL_SGRA13, L, g.stream);
+    else if (k.kind == K_QUAD && k.fused) e = launch_iter_quad(k.quad_n, L, g.stream);
     else if (k.kind == K_QUAD) e = launch_step_quad(k.quad_n, L, g.stream);
     else e = launch_step_user(L, g.stream);
     HIPCHK(e);
@@ -636,11 +642,17 @@ int launch_steps(Session* S, const Group& g) {
   return MHRE_OK;
 }
 
+// The exchange launch of group g (with no sequence to exchange, it only
+// hands the clock to the next iteration).
+int launch_group_exchange(Session* S, const Group& g) {
+  HIPCHK(launch_exchange(exchange_launch(S, g), g.d_x, g.xr_x, g.stream));
+  return MHRE_OK;
+}
+
 // One iteration of group g on its stream: the step launches, then the exchange.
 int launch_iteration(Session* S, const Group& g) {
   CHK(launch_steps(S, g));
-  HIPCHK(launch_exchange(exchange_launch(S, g), g.d_x, g.xr_x, g.stream));
-  return MHRE_OK;
+  return launch_group_exchange(S, g);
 }
 
 // The other groups' streams start after what the session stream holds so far
@@ -686,7 +698,7 @@ int time_kernels(Session* S, int niter, double* step_ms, double* exch_ms) {
         HIPCHK(hipEventRecord(e[0], g.stream));
         CHK(launch_steps(S, g));
         HIPCHK(hipEventRecord(e[1], g.stream));
-        HIPCHK(launch_exchange(exchange_launch(S, g), g.d_x, g.xr_x, g.stream));
+        CHK(launch_group_exchange(S, g));
         HIPCHK(hipEventRecord(e[2], g.stream));
       }
     }
@@ -1075,18 +1087,29 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
       const std::vector<int>& idx = by_kind[kk];
       if (idx.empty()) continue;
       if (kk == K_QUAD) {
-        // one kernel per compiled system size
+        // one kernel per compiled system size; ladders of at most 64 rungs
+        // run step and exchange in one launch (one item per wave: the fit
+        // and its first sequence), longer ones lane per chain
         for (int n : {2, 4, 8}) {
-          KindWork kw;
-          kw.kind = kk;
-          kw.quad_n = n;
+          KindWork kf, kw;
+          kf.kind = kw.kind = kk;
+          kf.quad_n = kw.quad_n = n;
+          kf.fused = true;
           for (int i : idx) {
             if (S->fits[i].quad_n != n) continue;
+            if (quad_fused(S->fits[i])) {
+              const int per = 64 / exchange_lanes(S->fits[i].s.nbeta);
+              int s0, s1;
+              seq_range(i, g, &s0, &s1);
+              for (int q = s0; q < s1; q += per) kf.step.push_back({i, q});
+              continue;
+            }
             const size_t before = kw.step.size();
             chains(i, kw.step);
             if (kw.step.size() > before)
               while (kw.step.size() % 64) kw.step.push_back({i, -1});
           }
+          if (!kf.step.empty()) grp.kinds.push_back(std::move(kf));
           if (!kw.step.empty()) grp.kinds.push_back(std::move(kw));
         }
         continue;
@@ -1109,6 +1132,7 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
     // one item per wave: its first sequence (64 / xlanes sequences per wave)
     std::vector<WorkItem> xw;
     for (int i : xorder) {
+      if (quad_fused(S->fits[i])) continue;   // exchanged inside its step launch
       const int per = 64 / exchange_lanes(S->fits[i].s.nbeta);
       int s0, s1;
       seq_range(i, g, &s0, &s1);

@@ -71,6 +71,9 @@ hipError_t launch_eval_eht(int model, const FitDesc* fits, const WorkItem* work,
 hipError_t launch_step_eht(int model, const StepLaunch& L, hipStream_t s);
 hipError_t launch_eval_quad(int n, const FitDesc* fits, const WorkItem* work, int nwork, hipStream_t s);
 hipError_t launch_step_quad(int n, const StepLaunch& L, hipStream_t s);
+// step + exchange of whole sequences (ladders of at most 64 rungs); work items
+// are (fit, first sequence of the wave)
+hipError_t launch_iter_quad(int n, const StepLaunch& L, hipStream_t s);
 hipError_t launch_exchange(const StepLaunch& L, const WorkItem* xwork, const XcdRanges& xr, hipStream_t s);
 
 // generic.hip: the library's compiled-in model.cuh (reference contract)

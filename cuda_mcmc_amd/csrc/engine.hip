@@ -496,24 +496,23 @@ struct QuadSys {
   static constexpr int QS = NS + NQ + 2;
   // qc: the fit's coefficients; the step kernel passes a wave-uniform
   // constant-space pointer (scalar loads), the evaluation kernel a global one
+  // Each equation as sum_i x_i (b_i + sum_{j>=i} c_ij x_j): NS + NQ FMAs
+  // and no table of the NS products x_i x_j, so a lane holds little more
+  // than x (the NQ inner sums are independent chains).
   template <class CP>
   __device__ static double chi2(CP qc, int neq, const float (&x)[NQ]) {
-    float xx[NS];
-    int m = 0;
-#pragma unroll
-    for (int i = 0; i < NQ; ++i)
-#pragma unroll
-      for (int j = i; j < NQ; ++j) xx[m++] = x[i] * x[j];
     double acc = 0.0;
     for (int e = 0; e < neq; ++e) {
       const CP c = qc + (size_t)e * QS;
-      // four interleaved partial sums keep the dependent FMA chain short
-      float l[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+      float lhs = 0.0f;
+      int m = 0;
 #pragma unroll
-      for (int i = 0; i < NS; ++i) l[i & 3] = fmaf(c[i], xx[i], l[i & 3]);
+      for (int i = 0; i < NQ; ++i) {
+        float t = c[NS + i];
 #pragma unroll
-      for (int i = 0; i < NQ; ++i) l[(NS + i) & 3] = fmaf(c[NS + i], x[i], l[(NS + i) & 3]);
-      const float lhs = (l[0] + l[1]) + (l[2] + l[3]);
+        for (int j = i; j < NQ; ++j) t = fmaf(c[m++], x[j], t);
+        lhs = fmaf(x[i], t, lhs);
+      }
       const float r = c[NS + NQ] - lhs;
       acc += (double)(r * r * c[NS + NQ + 1]);
     }
@@ -533,14 +532,18 @@ __global__ void __launch_bounds__(256) k_eval_quad(const FitDesc* fits, const Wo
   f.chi2c[w.idx] = (float)QuadSys<NQ>::chi2(gptr(f.qcoef), f.quad_neq, x);
 }
 
-// The chain's nprm proposals of this iteration (one lane); returns the
-// chi^2 evaluations performed.
-template <int NQ>
-__device__ unsigned quad_one_chain(const FitDesc& f, int ch, int it, const TraceBuf& tr) {
+// The chain's nprm proposals of this iteration (one lane) on the state x,
+// chi2c held by the caller; returns the chi^2 evaluations performed.
+// CACHE: in the burn-in stage the adaptation counters n_cnt and sumh of all
+// the chain's parameters are read once (one coalesced row per parameter) and
+// kept in registers for the iteration, instead of one scattered
+// read-modify-write per proposal (lanes pick different parameters).
+template <int NQ, bool CACHE>
+__device__ unsigned quad_substeps(const FitDesc& f, int ch, int it, const TraceBuf& tr, float (&x)[NQ],
+                                  float& chi2c) {
   const int C = f.nchain, P = f.nprm;
   const CONSTANT float* qc = wave_const(f.qcoef);
   const int neq = __builtin_amdgcn_readfirstlane(f.quad_neq);
-  GLOBAL float* pcur = gptr(f.pcur);
   GLOBAL float* pstp = gptr(f.pstp);
   GLOBAL int* n_acpt = gptr(f.n_acpt);
   GLOBAL int* n_cnt = gptr(f.n_cnt);
@@ -552,11 +555,15 @@ __device__ unsigned quad_one_chain(const FitDesc& f, int ch, int it, const Trace
   const CONSTANT float* pmaxt = wave_const(f.pmaxt);
   const int ibeta = ch / f.nseq;
   const bool burn = it < f.nburn;
-  float x[NQ];
-#pragma unroll
-  for (int i = 0; i < NQ; ++i) x[i] = pcur[(size_t)ch * NQ + i];
   const double beta = (double)gptr(f.beta)[ibeta];
-  float chi2c = gptr(f.chi2c)[ch];
+  int cnt_c[NQ], sh_c[NQ];
+  if (CACHE && burn) {
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+      cnt_c[i] = i < P ? n_cnt[(size_t)i * C + ch] : 0;
+      sh_c[i] = i < P ? sumh[(size_t)i * C + ch] : 0;
+    }
+  }
   int flag = 0, tlast = 0;
   float told = 0.0f;
   unsigned nev = 0;
@@ -601,19 +608,36 @@ __device__ unsigned quad_one_chain(const FitDesc& f, int ch, int it, const Trace
     told = old;
     tlast = t;
     if (burn) {
-      const int cnt = n_cnt[pk];
+      int cnt = 0, sh0 = 0;
+      if (CACHE) {
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+          cnt = (i == k) ? cnt_c[i] : cnt;
+          sh0 = (i == k) ? sh_c[i] : sh0;
+        }
+      } else {
+        cnt = n_cnt[pk];
+        sh0 = sumh[pk];
+      }
       const int a = cnt % f.nadj;   // oldest history entry, replaced by this outcome
       GLOBAL uint32_t* hw = hbits + ((size_t)(a >> 5) * P + k) * C + ch;
       const uint32_t word = *hw, bit = 1u << (a & 31);
       const int hold = (word & bit) ? 1 : 0;
       *hw = acc ? (word | bit) : (word & ~bit);
-      const int sh = sumh[pk] + acc - hold;
-      sumh[pk] = sh;
+      const int sh = sh0 + acc - hold;
       const int den = cnt < f.nadj ? cnt : f.nadj;
       const float rate = (float)sh / (float)den;
-      const float st = pstp[pk];
-      pstp[pk] = rate < f.adapt_target ? (float)((double)st / 1.01) : (float)((double)st * 1.01);
-      n_cnt[pk] = cnt + 1;
+      pstp[pk] = rate < f.adapt_target ? (float)((double)sd / 1.01) : (float)((double)sd * 1.01);
+      if (CACHE) {
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+          cnt_c[i] = (i == k) ? cnt + 1 : cnt_c[i];
+          sh_c[i] = (i == k) ? sh : sh_c[i];
+        }
+      } else {
+        sumh[pk] = sh;
+        n_cnt[pk] = cnt + 1;
+      }
     }
     if (tr.dec && it >= tr.it0 && it < tr.it0 + tr.nit) {
       const size_t ti = (size_t)f.tr_dec + ((size_t)(it - tr.it0) * P + s) * C + ch;
@@ -621,21 +645,44 @@ __device__ unsigned quad_one_chain(const FitDesc& f, int ch, int it, const Trace
       gptr(tr.chi2)[ti] = inprior ? (float)c2 : 0.0f;
     }
   }
+  if (CACHE && burn) {
 #pragma unroll
-  for (int i = 0; i < NQ; ++i) pcur[(size_t)ch * NQ + i] = x[i];
-  gptr(f.chi2c)[ch] = chi2c;
+    for (int i = 0; i < NQ; ++i) {
+      if (i < P) {
+        n_cnt[(size_t)i * C + ch] = cnt_c[i];
+        sumh[(size_t)i * C + ch] = sh_c[i];
+      }
+    }
+  }
   gptr(f.flag)[ch] = flag;
   gptr(f.ptent)[ch] = told;
   gptr(f.ptentn)[ch] = tlast;
   return nev;
 }
 
+template <int NQ>
+__device__ unsigned quad_one_chain(const FitDesc& f, int ch, int it, const TraceBuf& tr) {
+  GLOBAL float* pcur = gptr(f.pcur);
+  float x[NQ];
+#pragma unroll
+  for (int i = 0; i < NQ; ++i) x[i] = pcur[(size_t)ch * NQ + i];
+  float chi2c = gptr(f.chi2c)[ch];
+  const unsigned nev = quad_substeps<NQ, false>(f, ch, it, tr, x, chi2c);
+#pragma unroll
+  for (int i = 0; i < NQ; ++i) pcur[(size_t)ch * NQ + i] = x[i];
+  gptr(f.chi2c)[ch] = chi2c;
+  return nev;
+}
+
 
 // One lane per chain; all chains of a wave belong to one fit (the host pads
 // every fit's chain list to whole waves), so the coefficients are uniform.
+// Registers are capped so that 8 waves fit on a SIMD: the lane-per-chain
+// work waits on the scalar coefficient loads and the state loads, which more
+// resident waves hide.
 template <int NQ>
-__global__ void __launch_bounds__(256) k_step_quad(const FitDesc* fits, const WorkItem* work, int nwork,
-                                                   RunClock* clk, TraceBuf tr) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
+k_step_quad(const FitDesc* fits, const WorkItem* work, int nwork, RunClock* clk, TraceBuf tr) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (__builtin_amdgcn_readfirstlane(g) >= nwork) return;   // whole waves (the list is padded to 64)
   const WorkItem w = work[g];
@@ -787,6 +834,113 @@ __global__ void __launch_bounds__(64) k_exchange(const FitDesc* fits, const Work
   }
 }
 
+// ------------------------------------- quadratic systems: step + exchange
+
+// One whole MH-RE iteration of quadratic-system sequences in one launch: the
+// lane-per-chain proposals of k_step_quad, then the replica exchange of every
+// sequence among the lanes that hold its temperature slots, with the chain
+// states still in registers (parameter rows move by lane shuffles, not through
+// memory).  A wave holds G = 64 / W sequences (W = xlanes, ladders of at most
+// 64 rungs); lane l is temperature slot l / G of sequence iseq0 + l % G, so
+// the G sequences of one slot are adjacent in the state arrays.  The random
+// counters are those of k_step_quad and k_exchange (reference update rule:
+// gpu_mcmc.cu:361-526 then :533-626), so the chains are the same bit for bit.
+// The burn-in counters held in registers (quad_substeps<NQ, true>) take the
+// register cap to 7 waves per SIMD.
+// The body of k_iter_quad for the wave's G sequences from iseq0.
+template <int NQ>
+__device__ void iter_quad_wave(const FitDesc& f, int iseq0, int it, const TraceBuf& tr) {
+  const int lane = lane_id();
+  const int B = f.nbeta, S = f.nseq, W = f.xlanes;
+  const int lg = 6 - __builtin_ctz((unsigned)W);   // log2 G
+  const int G = 1 << lg;
+  const int sl = lane >> lg, g = lane & (G - 1);
+  const int iseq = __builtin_amdgcn_readfirstlane(iseq0) + g;
+  const bool seq_ok = iseq < S;
+  const bool slot = seq_ok && sl < B;
+  const int ch = sl * S + iseq;
+  const bool burn = it < f.nburn;
+  GLOBAL float* pcur = gptr(f.pcur);
+  GLOBAL float* chi2c_a = gptr(f.chi2c);
+  GLOBAL int* tcur_a = gptr(f.tcur);
+  float x[NQ];
+  float c2 = 0.0f;
+  int tc = 0;
+#pragma unroll
+  for (int i = 0; i < NQ; ++i) x[i] = slot ? pcur[(size_t)ch * NQ + i] : 0.0f;
+  if (slot) {
+    c2 = chi2c_a[ch];
+    tc = tcur_a[ch];
+  }
+  const unsigned nev = slot ? quad_substeps<NQ, true>(f, ch, it, tr, x, c2) : 0u;
+  wave_count(f, nev, slot ? (unsigned)f.nprm : 0u);
+
+  // exchange: as exchange_wave, with the partner of slot k at lane k * G + g
+  int perm = sl;
+  int nex = 0;
+  if (B > 1) {
+    int k_l = 0;
+    double lu_l = 0.0, hb_l = 0.0;
+    if (slot) {
+      const U4 r = philox4x32_10((uint32_t)iseq, (uint32_t)it, (uint32_t)sl, RNG_EXCH, f.key0, f.key1);
+      k_l = pick_with_retry(r.x, B - 1, (uint32_t)iseq, (uint32_t)it, (uint32_t)sl, RNG_EXCH, f.key0, f.key1);
+      lu_l = log((double)u01(r.y));
+    }
+    if (sl < B - 1) hb_l = -0.5 * ((double)f.beta[sl + 1] - (double)f.beta[sl]);
+    for (int a = 0; a < B; ++a) {   // every lane takes part in every shuffle
+      const int k = __shfl(k_l, (a << lg) + g);
+      const int lk = (k << lg) + g, lk1 = lk + G;
+      const float ck = __shfl(c2, lk), ck1 = __shfl(c2, lk1);
+      const double xx = __shfl(hb_l, lk) * ((double)ck - (double)ck1);
+      const bool acc = __shfl(lu_l, (a << lg) + g) <= xx;
+      const int tk = __shfl(tc, lk), tk1 = __shfl(tc, lk1);
+      const int pk = __shfl(perm, lk), pk1 = __shfl(perm, lk1);
+      if (acc) {
+        if (sl == k) { c2 = ck1; tc = tk1; perm = pk1; if (!burn) ++nex; }
+        if (sl == k + 1) { c2 = ck; tc = tk; perm = pk; }
+      }
+      if (tr.ex && sl == 0 && seq_ok && it >= tr.it0 && it < tr.it0 + tr.nit)
+        tr.ex[(size_t)f.tr_ex + ((size_t)(it - tr.it0) * B + a) * S + iseq] = k | ((acc ? 1 : 0) << 16);
+    }
+  }
+  // slot lane receives the row that started in slot perm
+  float row[NQ];
+  const int src = (perm << lg) + g;
+#pragma unroll
+  for (int i = 0; i < NQ; ++i) row[i] = __shfl(x[i], src);
+  if (slot) {
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) pcur[(size_t)ch * NQ + i] = row[i];
+    chi2c_a[ch] = c2;
+    tcur_a[ch] = tc;
+    if (!burn && nex) gptr(f.n_exch)[ch] += nex;
+  }
+  if (!burn && f.record) {
+    const int itr = it - f.nburn;
+    if (slot) {
+#pragma unroll
+      for (int i = 0; i < NQ; ++i) {
+        const int ip = f.invar[i];
+        if (ip >= 0) f.pout[(((size_t)B * ip + sl) * S + iseq) * f.niter + itr] = row[i];
+      }
+      f.tout[(size_t)ch * f.niter + itr] = tc;
+    }
+    if (sl == 0 && seq_ok) f.chi2[(size_t)iseq * f.niter + itr] = c2;
+  }
+}
+
+template <int NQ>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7)))
+k_iter_quad(const FitDesc* fits, const WorkItem* work, int nwork, RunClock* clk, TraceBuf tr) {
+  const int wv = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
+  if (wv >= nwork) return;
+  pass_clock(clk);
+  const WorkItem w = work[wv];
+  const FitDesc& f = wave_fit(fits, w.fit);
+  const int it = __builtin_amdgcn_readfirstlane(load_clock(clk));
+  if (it < f.nburn + f.niter) iter_quad_wave<NQ>(f, w.idx, it, tr);
+}
+
 // ---------------------------------------------------------------- launchers
 
 static inline int cdiv(long a, long b) { return (int)((a + b - 1) / b); }
@@ -889,10 +1043,24 @@ hipError_t launch_step_quad(int n, const StepLaunch& L, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_exchange(const StepLaunch& L, const WorkItem* xwork, const XcdRanges& xr, hipStream_t s) {
-  if (xr.off[NXCD] <= 0) return hipSuccess;
+hipError_t launch_iter_quad(int n, const StepLaunch& L, hipStream_t s) {
+  if (L.nwork <= 0) return hipSuccess;
   const TraceBuf tr{L.tr_dec, L.tr_chi2, L.tr_ex, L.tr_it0, L.tr_nit};
-  k_exchange<<<xcd_grid(xr, 1), 64, 0, s>>>(L.fits, xwork, xr, L.clk, tr);
+  const int g = cdiv(L.nwork, 4);   // one work item per wave, 4 waves per work-group
+  switch (n) {
+    case 2: k_iter_quad<2><<<g, 256, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr); break;
+    case 4: k_iter_quad<4><<<g, 256, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr); break;
+    case 8: k_iter_quad<8><<<g, 256, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr); break;
+    default: return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+// With no sequence left to exchange (every fit's exchange ran inside its step
+// launch) one work-group still runs, to hand the clock to the next iteration.
+hipError_t launch_exchange(const StepLaunch& L, const WorkItem* xwork, const XcdRanges& xr, hipStream_t s) {
+  const TraceBuf tr{L.tr_dec, L.tr_chi2, L.tr_ex, L.tr_it0, L.tr_nit};
+  k_exchange<<<xr.off[NXCD] > 0 ? xcd_grid(xr, 1) : 1, 64, 0, s>>>(L.fits, xwork, xr, L.clk, tr);
   return hipGetLastError();
 }
 

@@ -112,7 +112,11 @@ struct XcdRanges {
 //   step launch:     it = clk.it;  one thread stores clk.xit = it
 //   exchange launch: it = clk.xit; one thread stores clk.it = it + 1 and
 //                    clears the queue heads for the next step launches
-// so no launch has to count its finished work-groups.
+// so no launch has to count its finished work-groups.  (A sequence group
+// whose fits all exchange inside their step launch, k_iter_quad, still ends
+// its iteration with an exchange launch, of one work-group, for the clock:
+// counting the step launch's finished waves in one device-scope atomic
+// instead made those launches slower by more than that launch costs.)
 constexpr int NQSLOT = 6;
 struct RunClock {
   int it;                     // iteration of the next step launches (burn-in first, then sampling)
