@@ -20,6 +20,8 @@ Also reported:
                 157.3 TFLOP/s peak; also the average duration of one step
                 launch and one exchange launch (HIP events around each);
   cpu_baseline  the float64 C/OpenMP oracle port on the host cores;
+  pcie_inclusive the same whole fit through host buffers (upload, run,
+                download of the recorded samples), host wall time;
   batched       64 uneven fits (500-20000 visibilities, 16-48 x 64-256
                 chains) in one session, launch form picked by the
                 autotuner, vs the same fits one
@@ -70,7 +72,7 @@ def eht_fit(d, nbeta, nseq, nburn, niter, seed):
 
 def traffic_from_profiles():
     """HBM bytes per step launch from the committed PMC summary, if present."""
-    p = os.path.join(ROOT, "profiles", "sgra9_step_config2_pmc.txt")
+    p = os.path.join(ROOT, "profiles", "sgra9_step_4096chains_2000vis_pmc.txt")
     if not os.path.isfile(p):
         return None
     for line in open(p):
@@ -306,6 +308,7 @@ def main():
     ap.add_argument("--cpu-iters", type=int, default=8)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-quadsys", action="store_true")
+    ap.add_argument("--no-pcie", action="store_true")
     ap.add_argument("--launch-probe", type=int, default=20,
                     help="iterations of the per-launch timing probe (0: no probe session)")
     args = ap.parse_args()
@@ -419,6 +422,20 @@ def main():
                      "launch_probe_iterations": [p0, p0 + nprobe] if nprobe else None,
                      "flop_per_iteration": flop / max(steps, 1)},
     }
+    if not args.no_pcie:
+        # the same whole fit through the host-buffer boundary: session
+        # creation (upload), every iteration, and the download of the
+        # recorded samples, timed on the host
+        from cuda_mcmc_amd.engine import run_fits
+        hf = eht_fit(d, 32, 128, nburn=nburn_cfg, niter=niter_cfg, seed=0)
+        t0 = time.time()
+        st = run_fits([hf], **opts)
+        tw = time.time() - t0
+        nbytes = sum(getattr(hf, k).nbytes for k in ("pout", "tout", "chi2", "pcur", "n_hist"))
+        out["pcie_inclusive"] = {"value": round(st["vis_evals"] / tw / 1e9, 3), "unit": "G visibility-evals/s",
+                                 "seconds": round(tw, 3), "iterations": total_cfg,
+                                 "download_bytes": int(nbytes)}
+        log("host buffers in and out: %.3f s for the whole fit" % tw)
     if not args.no_batched:
         log("batched vs plain (%d fits)..." % args.batched_fits)
         out["batched"] = batched_vs_plain(args.batched_iters, args.batched_fits)
