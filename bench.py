@@ -123,7 +123,8 @@ def cpu_baseline(d, state, it0, iters, nthreads):
 
 
 def batched_vs_plain(iters, nfits, log_every=16):
-    """Config-3 style batch: uneven fits in one session vs one session per fit."""
+    """64 uneven fits (500-20000 visibilities, 5-10 stations, 16-48 x 64-256
+    chains) in one session vs one session per fit."""
     from cuda_mcmc_amd import synthdata
     from cuda_mcmc_amd.engine import Session
     fits, datas = [], []
@@ -198,7 +199,8 @@ def batched_vs_plain(iters, nfits, log_every=16):
 
 
 def adjoint_leg(npts=4096, nvis=20000, ncph=6000, reps=5, gn_fits=64, gn_burn=1000, gn_iter=250):
-    """Config 4: batched chi^2 gradient throughput, then GN polish + Laplace."""
+    """chi^2 + gradient of 4096 points against 20000 visibilities + 6000
+    closures, then Gauss-Newton polish + Laplace covariance of 64 fits."""
     import torch
     from cuda_mcmc_amd import adjoint, synthdata
     from cuda_mcmc_amd.engine import Session
@@ -225,7 +227,7 @@ def adjoint_leg(npts=4096, nvis=20000, ncph=6000, reps=5, gn_fits=64, gn_burn=10
     out = dict(points=npts, nvis=nvis, ncph=ncph, ms_per_call=round(ms, 4), ms_chi2_only=round(ms_fwd, 4),
                vis_grad_evals_per_s=npts * nvis / (ms * 1e-3),
                model_evals_per_s=npts * (nvis + 3 * ncph) / (ms * 1e-3))
-    # 64 short fits of the config-2 shape sampled in one batched session,
+    # 64 short fits of the plain-fit shape sampled in one batched session,
     # then Gauss-Newton on the best chain of each and its Laplace covariance
     before, after, ratios = [], [], []
     dsets = [synthdata.ring_gauss_dataset(nvis=2000, ncph=600, nsta=8, seed=500 + f) for f in range(gn_fits)]
@@ -260,7 +262,8 @@ def adjoint_leg(npts=4096, nvis=20000, ncph=6000, reps=5, gn_fits=64, gn_burn=10
 
 
 def quadsys_leg(nsys=1024, nburn=3000, niter=10000):
-    """Config 5: every quadratic system one fit of one session, then GN polish."""
+    """1024 systems of 8 quadratic equations, every system one fit of one
+    session, then Gauss-Newton polish."""
     from cuda_mcmc_amd import quadsys, synthdata
     from cuda_mcmc_amd.engine import Session
     ds = synthdata.quadsys_dataset(nsys=nsys, seed=5)

@@ -1,10 +1,10 @@
 """Batched sessions against the oracle: uneven fits in one session, each fit
 held to the reference's per-fit update rule (src/gpu_mcmc.cu:361-626).
 
-The fits differ in data size (600 to 5000 visibilities with closure phases at
-30 % of that), station count, ladder (16, 20, 33 and 48 rungs; 16 and 20 share
-an exchange wave between sequences, 33 and 48 take a wave each) and number of
-sequences (64 to 128).  Every iteration each fit's oracle restarts from the
+The fits differ in data size (600 to 20000 visibilities with closure phases
+at 30 % of that), station count, ladder (16, 20, 24, 33 and 48 rungs; 16, 20
+and 24 share an exchange wave between sequences, 33 and 48 take a wave each)
+and number of sequences (8 to 128).  Every iteration each fit's oracle restarts from the
 state the device downloaded and runs the same iteration; picks must be
 identical, accept and swap decisions identical outside the tie band, and the
 state of every sequence without a near-tie equal.  Both launch forms of the
@@ -25,7 +25,8 @@ pytestmark = pytest.mark.gpu
 FITS = [(5000, 1500, 8, 16, 64, 301),
         (1200, 360, 6, 33, 64, 302),
         (600, 180, 5, 48, 80, 303),
-        (900, 270, 10, 20, 128, 304)]
+        (900, 270, 10, 20, 128, 304),
+        (20000, 6000, 10, 24, 8, 305)]
 
 
 def _batch():
