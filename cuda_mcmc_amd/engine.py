@@ -184,7 +184,10 @@ class Session:
         _lib.check(self.L, self.L.mhre_session_run(self.h, int(niter)), "mhre_session_run")
 
     def time_kernels(self, niter):
-        """Run niter iterations with per-launch HIP events; (step_ms, exchange_ms) per iteration."""
+        """Run niter iterations with HIP events around every launch; returns
+        (step_ms, exchange_ms): one sequence group's step and exchange launch
+        durations per iteration (groups overlap, so they do not add up to the
+        iteration time, which stats()["run_ms"] gives after run())."""
         a, b = C.c_double(), C.c_double()
         _lib.check(self.L, self.L.mhre_session_time_kernels(self.h, int(niter), C.byref(a), C.byref(b)),
                    "mhre_session_time_kernels")

@@ -159,9 +159,11 @@ typedef struct MhreStats {
 int mhre_session_create(const MhreFitSpec *fits, int nfits, const MhreOptions *opt, void **session);
 /* Run niter MH-RE iterations of every fit (asynchronous to the host). */
 int mhre_session_run(void *session, int niter);
-/* Run niter iterations timing every launch with HIP events on the session's
- * stream; returns the average device time per iteration of the step
- * kernel(s) and of the exchange kernel, in milliseconds. */
+/* Run niter iterations timing every launch with HIP events on the stream of
+ * its sequence group; returns the average device time of one group's step
+ * launch(es) and of its exchange launch per iteration, in milliseconds (the
+ * launches of different groups overlap, so these do not add up to the
+ * iteration time). */
 int mhre_session_time_kernels(void *session, int niter, double *step_ms, double *exch_ms);
 /* Choose the step kernels' launch form (waves per work-group 1/2/4, grid or
  * persistent work queue) by timing each candidate on `iters` real
