@@ -56,9 +56,10 @@ def test_lockstep_solvsys_generic():
     assert compared >= 8 * 14 * 0.5
 
 
-@pytest.mark.parametrize("nbeta,nseq", [(16, 32), (70, 16)])
+@pytest.mark.parametrize("nbeta,nseq", [(16, 32), (1, 100), (70, 16)])
 def test_lockstep_quadsys_native(nbeta, nseq):
-    # 16 rungs: proposals and exchange in one launch (k_iter_quad); 70 rungs:
+    # 16 rungs: proposals and exchange in one launch (k_iter_quad); one rung:
+    # the same kernel with 64 sequences per wave and no swaps; 70 rungs:
     # lane-per-chain step launch, then the exchange launch.  A 70-rung
     # sequence makes 630 decisions per iteration, so few sequences get through
     # 7 iterations without a single one in the tie band; their decisions are
