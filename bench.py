@@ -122,9 +122,10 @@ def cpu_baseline(d, state, it0, iters, nthreads):
                        % (iters, it0, it0 + iters - 1, nev, nthreads, dt))
 
 
-def batched_vs_plain(iters, nfits, log_every=16):
+def batched_vs_plain(iters, nfits, log_every=16, session_only=False):
     """64 uneven fits (500-20000 visibilities, 5-10 stations, 16-48 x 64-256
-    chains) in one session vs one session per fit."""
+    chains) in one session vs one session per fit (session_only: the batched
+    session alone, for profiling)."""
     from cuda_mcmc_amd import synthdata
     from cuda_mcmc_amd.engine import Session
     fits, datas = [], []
@@ -157,6 +158,10 @@ def batched_vs_plain(iters, nfits, log_every=16):
         total_vis = e1["vis_evals"] - e0["vis_evals"]
         tb_dev = e1["run_ms"] * 1e-3
     batched = total_vis / tb
+    if session_only:
+        return dict(fits=nfits, iterations=iters, streams=2, autotuned={"waves_per_block": wpb, "persistent": pers},
+                    batched_gvis_per_s=batched / 1e9, batched_device_gvis_per_s=total_vis / tb_dev / 1e9,
+                    iteration_ms=tb_dev * 1e3 / iters)
     tp = 0.0
     plain_vis = 0.0
     for f, (d, nb, ns) in enumerate(fits):
@@ -307,6 +312,8 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-fits", type=int, default=64)
     ap.add_argument("--batched-iters", type=int, default=16)
+    ap.add_argument("--batched-session-only", action="store_true",
+                    help="batched leg: only the one-session run (no per-fit comparison sessions)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--cpu-iters", type=int, default=8)
     ap.add_argument("--no-adjoint", action="store_true")
@@ -441,7 +448,8 @@ def main():
         log("host buffers in and out: %.3f s for the whole fit" % tw)
     if not args.no_batched:
         log("batched vs plain (%d fits)..." % args.batched_fits)
-        out["batched"] = batched_vs_plain(args.batched_iters, args.batched_fits)
+        out["batched"] = batched_vs_plain(args.batched_iters, args.batched_fits,
+                                          session_only=args.batched_session_only)
     if not args.no_adjoint:
         log("adjoint gradient + Gauss-Newton...")
         out["adjoint"] = adjoint_leg()
