@@ -715,11 +715,6 @@ __device__ void exchange_wave(const FitDesc& f, int iseq0, int it, bool burn, co
   const size_t me = (size_t)sl * S + iseq;
   float c2 = slot ? f.chi2c[me] : 0.0f;
   int tc = slot ? f.tcur[me] : 0;
-  // the slot's own parameter row, read while the attempts run: the rows then
-  // move between lanes by shuffles, not by a second, dependent read
-  float row[MAXP];
-#pragma unroll
-  for (int j = 0; j < MAXP; ++j) row[j] = (slot && j < N) ? f.pcur[me * N + j] : 0.0f;
   int perm = sl;
   int nex = 0;
   if (B > 1) {
@@ -749,10 +744,10 @@ __device__ void exchange_wave(const FitDesc& f, int iseq0, int it, bool burn, co
     }
   }
   // move parameter rows: slot lane receives the row that started in slot perm
-  // (every lane takes part in the shuffles; j < N is the same on all lanes)
+  float row[MAXP];
 #pragma unroll
   for (int j = 0; j < MAXP; ++j)
-    if (j < N) row[j] = __shfl(row[j], gb + perm);
+    row[j] = (slot && j < N) ? f.pcur[((size_t)perm * S + iseq) * N + j] : 0.0f;
   if (slot && perm != sl) {
 #pragma unroll
     for (int j = 0; j < MAXP; ++j)
