@@ -86,8 +86,9 @@ def traffic_from_profiles():
 
 def headline_options(args):
     """Launch form of the plain fit's session: 4 waves per work-group, one
-    wave per chain, the 128 sequences in two groups on two streams."""
-    return dict(waves_per_block=4, persistent=0, streams=2)
+    wave per chain, the 128 sequences in two groups on two streams
+    (--streams changes the group count, e.g. 1 for hardware-counter runs)."""
+    return dict(waves_per_block=4, persistent=0, streams=args.streams)
 
 
 def host_threads():
@@ -319,6 +320,7 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-quadsys", action="store_true")
     ap.add_argument("--no-pcie", action="store_true")
+    ap.add_argument("--streams", type=int, default=2, help="sequence groups of the headline session")
     ap.add_argument("--launch-probe", type=int, default=20,
                     help="iterations of the per-launch timing probe (0: no probe session)")
     args = ap.parse_args()
@@ -357,7 +359,7 @@ def main():
     fit = eht_fit(d, 32, 128, nburn=nburn_cfg, niter=niter_cfg, seed=0)
     opts = dict(headline_options(args))
     # sequence groups of the session (the engine caps them at the 64 exchange waves)
-    ngroups = max(1, min(opts.get("streams", 1), 8, 64))
+    ngroups = max(1, min(opts.get("streams", 1) or 2, 8, 64))
     with Session([fit], **opts) as ses:
         if w0 > 0:
             ses.run(w0)
