@@ -46,6 +46,34 @@ def best_rows(fits):
     return np.array(X, np.float32), np.array(c2, np.float64)
 
 
+def best_sequence_rows(fits, k=4):
+    """Starting points from independent sequences: the lowest-chi^2 chain of
+    each of the k sequences whose best chain is lowest, per fit.  Returns
+    X [nfits * k, n], chi^2 [nfits * k] and the fit of every row."""
+    X, c2, sys = [], [], []
+    for s, f in enumerate(fits):
+        ib = np.argmin(f.chi2c, axis=0)                   # best temperature slot of every sequence
+        best = f.chi2c[ib, np.arange(f.nseq)]
+        for q in np.argsort(best, kind="stable")[:k]:
+            X.append(f.pcur[ib[q], q])
+            c2.append(best[q])
+            sys.append(s)
+    return np.array(X, np.float32), np.array(c2, np.float64), np.array(sys, np.int32)
+
+
+def polish_best(Q, b, dat, X, sys, nsys, iters=20, lib=None):
+    """Gauss-Newton from every row of X (row i belongs to system sys[i]);
+    returns the lowest-residual result of each system: X [nsys, n], |r| [nsys]."""
+    Xp, nr = gauss_newton(Q, b, dat, X, iters=iters, sys=sys, lib=lib)
+    best = np.full(nsys, np.inf)
+    out = np.zeros((nsys, X.shape[1]), np.float32)
+    for i in range(len(sys)):
+        if nr[i] < best[sys[i]]:
+            best[sys[i]] = nr[i]
+            out[sys[i]] = Xp[i]
+    return out, best
+
+
 def jacobian(Q, b, dat, X, sys=None, lib=None):
     """Residuals r [rows, neq] = x^T Q_e x + b_e . x - dat_e and J [rows, neq, n] on the GPU."""
     L = _lib.load(lib)
@@ -67,15 +95,16 @@ def jacobian(Q, b, dat, X, sys=None, lib=None):
     return r, J
 
 
-def gauss_newton(Q, b, dat, X0, iters=20, lam=1e-6, box=(-5.0, 5.0), lib=None):
-    """Levenberg-Marquardt polish of X0 [nsys, n] (row i solves system i).
+def gauss_newton(Q, b, dat, X0, iters=20, lam=1e-6, box=(-5.0, 5.0), sys=None, lib=None):
+    """Levenberg-Marquardt polish of X0 [rows, n] (row i solves system sys[i],
+    or system i when sys is None).
 
     Returns (X, |r|) with |r| the residual norm of every row (float32 device
     evaluation, float64 accumulation).
     """
     X = np.asarray(X0, np.float64).copy()
     lamv = np.full(X.shape[0], float(lam))
-    r, J = jacobian(Q, b, dat, X, lib=lib)
+    r, J = jacobian(Q, b, dat, X, sys=sys, lib=lib)
     nr = np.linalg.norm(r.astype(np.float64), axis=1)
     for _ in range(iters):
         Jd = J.astype(np.float64)
@@ -85,7 +114,7 @@ def gauss_newton(Q, b, dat, X0, iters=20, lam=1e-6, box=(-5.0, 5.0), lib=None):
             np.einsum("skk->sk", JtJ)[:, :, None], 1e-12)
         step = -np.linalg.solve(A, Jtr[..., None])[..., 0]
         trial = np.clip(X + step, box[0], box[1])
-        r2, J2 = jacobian(Q, b, dat, trial, lib=lib)
+        r2, J2 = jacobian(Q, b, dat, trial, sys=sys, lib=lib)
         n2 = np.linalg.norm(r2.astype(np.float64), axis=1)
         ok = n2 < nr
         X[ok], r[ok], J[ok], nr[ok] = trial[ok], r2[ok], J2[ok], n2[ok]

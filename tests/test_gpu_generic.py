@@ -86,6 +86,22 @@ def test_quad_gauss_newton_from_near_root():
     np.testing.assert_allclose(X, ds["xstar"], atol=1e-4)
 
 
+def test_quad_polish_keeps_the_best_start_per_system():
+    # two starts per system, one next to the planted root, one anywhere in
+    # the box: the polish keeps the lower residual of the two
+    ds = synthdata.quadsys_dataset(nsys=8, seed=12)
+    rng = np.random.default_rng(1)
+    good = ds["xstar"] + 0.02 * rng.standard_normal(ds["xstar"].shape)
+    far = rng.uniform(-5, 5, ds["xstar"].shape)
+    X = np.concatenate([far, good]).astype(np.float32)
+    sysv = np.concatenate([np.arange(8), np.arange(8)]).astype(np.int32)
+    Xb, r = quadsys.polish_best(ds["Q"], ds["b"], -ds["c"], X, sysv, 8, iters=25)
+    _, r_good = quadsys.gauss_newton(ds["Q"], ds["b"], -ds["c"], good.astype(np.float32), iters=25)
+    _, r_far = quadsys.gauss_newton(ds["Q"], ds["b"], -ds["c"], far.astype(np.float32), iters=25)
+    np.testing.assert_array_equal(r, np.minimum(r_good, r_far))
+    assert np.all(r < 1e-5)
+
+
 def test_quad_solve_end_to_end():
     ds = synthdata.quadsys_dataset(nsys=32, seed=9)
     out = quadsys.solve_systems(ds, nbeta=16, nseq=32, nburn=1500, gn_iters=25, seed=100)

@@ -65,3 +65,20 @@ def test_interface_checks_dtype_shape_contiguity():
         a = np.zeros((2, 9), f32)[:, ::2]
         mi.calcmodel(np.zeros(4, f32), np.zeros(3, np.int32), np.zeros(8, f32), a.reshape(1, 2, -1),
                      1, 4, 2, 3, 8, 9, 1, 1)
+
+
+def test_best_sequence_rows_takes_each_sequences_best_chain():
+    from types import SimpleNamespace
+    from cuda_mcmc_amd import quadsys
+    B, S, N = 3, 5, 2
+    chi2c = np.array([[5., 1., 9., 4., 8.],
+                      [2., 7., 3., 6., 0.5],
+                      [6., 2., 1., 5., 9.]], np.float32)
+    pcur = np.arange(B * S * N, dtype=np.float32).reshape(B, S, N)
+    f = SimpleNamespace(chi2c=chi2c, pcur=pcur, nseq=S)
+    X, c2, sysv = quadsys.best_sequence_rows([f, f], k=3)
+    # per sequence best: s0 2.0 (b1), s1 1.0 (b0), s2 1.0 (b2), s3 4.0 (b0), s4 0.5 (b1)
+    # three lowest: s4 (0.5), s1 (1.0), s2 (1.0; stable order after s1)
+    np.testing.assert_array_equal(c2, [0.5, 1.0, 1.0] * 2)
+    np.testing.assert_array_equal(X[:3], [pcur[1, 4], pcur[0, 1], pcur[2, 2]])
+    np.testing.assert_array_equal(sysv, [0, 0, 0, 1, 1, 1])
