@@ -123,7 +123,7 @@ def cpu_baseline(d, state, it0, iters, nthreads):
                        % (iters, it0, it0 + iters - 1, nev, nthreads, dt))
 
 
-def batched_vs_plain(iters, nfits, log_every=16, session_only=False):
+def batched_vs_plain(iters, nfits, log_every=16, session_only=False, streams=2):
     """64 uneven fits (500-20000 visibilities, 5-10 stations, 16-48 x 64-256
     chains) in one session vs one session per fit (session_only: the batched
     session alone, for profiling)."""
@@ -146,7 +146,7 @@ def batched_vs_plain(iters, nfits, log_every=16, session_only=False):
     tune_iters = 2
     mk = lambda d, nb, ns, f: eht_fit(d, nb, ns, nburn=iters + 2 + 6 * tune_iters, niter=0, seed=300 + f)
     total_vis = 0.0
-    with Session([mk(d, nb, ns, f) for f, (d, nb, ns) in enumerate(fits)], persistent=0, streams=2) as s:
+    with Session([mk(d, nb, ns, f) for f, (d, nb, ns) in enumerate(fits)], persistent=0, streams=streams) as s:
         s.run(1)
         wpb, pers, _ = s.autotune(tune_iters)
         s.sync()
@@ -157,12 +157,16 @@ def batched_vs_plain(iters, nfits, log_every=16, session_only=False):
         tb = time.time() - t0
         e1 = s.stats()
         total_vis = e1["vis_evals"] - e0["vis_evals"]
+        total_ev = e1["chi2_evals"] - e0["chi2_evals"]
+        total_cph = e1["cph_evals"] - e0["cph_evals"]
         tb_dev = e1["run_ms"] * 1e-3
     batched = total_vis / tb
     if session_only:
-        return dict(fits=nfits, iterations=iters, streams=2, autotuned={"waves_per_block": wpb, "persistent": pers},
+        return dict(fits=nfits, iterations=iters, streams=streams,
+                    autotuned={"waves_per_block": wpb, "persistent": pers},
                     batched_gvis_per_s=batched / 1e9, batched_device_gvis_per_s=total_vis / tb_dev / 1e9,
-                    iteration_ms=tb_dev * 1e3 / iters)
+                    iteration_ms=tb_dev * 1e3 / iters, vis_evals=total_vis, cph_evals=total_cph,
+                    chi2_evals=total_ev)
     tp = 0.0
     plain_vis = 0.0
     for f, (d, nb, ns) in enumerate(fits):
@@ -194,7 +198,7 @@ def batched_vs_plain(iters, nfits, log_every=16, session_only=False):
             tg += time.time() - t0
             gen_vis += s.stats()["vis_evals"] - e0["vis_evals"]
     generic = gen_vis / tg
-    return dict(fits=nfits, iterations=iters, streams=2, autotuned={"waves_per_block": wpb, "persistent": pers},
+    return dict(fits=nfits, iterations=iters, streams=streams, autotuned={"waves_per_block": wpb, "persistent": pers},
                 batched_device_gvis_per_s=total_vis / tb_dev / 1e9,
                 plain_generic_sequential_gvis_per_s=generic / 1e9, ratio_vs_generic=batched / generic,
                 chains=int(sum(nb * ns for _, nb, ns in fits)),
@@ -327,7 +331,8 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-quadsys", action="store_true")
     ap.add_argument("--no-pcie", action="store_true")
-    ap.add_argument("--streams", type=int, default=2, help="sequence groups of the headline session")
+    ap.add_argument("--streams", type=int, default=2,
+                    help="sequence groups of the headline and batched sessions")
     ap.add_argument("--launch-probe", type=int, default=20,
                     help="iterations of the per-launch timing probe (0: no probe session)")
     args = ap.parse_args()
@@ -458,7 +463,7 @@ def main():
     if not args.no_batched:
         log("batched vs plain (%d fits)..." % args.batched_fits)
         out["batched"] = batched_vs_plain(args.batched_iters, args.batched_fits,
-                                          session_only=args.batched_session_only)
+                                          session_only=args.batched_session_only, streams=args.streams)
     if not args.no_adjoint:
         log("adjoint gradient + Gauss-Newton...")
         out["adjoint"] = adjoint_leg()
