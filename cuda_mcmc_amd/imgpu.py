@@ -206,18 +206,17 @@ class Mcgpu_Sgra(Mcgpu):
             if np.all(self.fsig != 0):
                 self.eamp = np.copy(self.fsig)
             else:
-                return self._fail("Some sigmas in uv file are zeros.")
+                return self._fail("the uv file gives a zero sigma for some visibilities")
         elif np.isscalar(eamp):
             if eamp == 0:
-                return self._fail("1-sigma visibility amplitude error eamp is zero.")
+                return self._fail("eamp (the amplitudes' 1-sigma error) is zero")
             self.eamp[:] = eamp
         elif len(eamp) == len(self.amp):
             if np.any(np.asarray(eamp) == 0):
-                return self._fail("Some of eamp elements are zero.")
+                return self._fail("eamp has zero elements")
             self.eamp = np.asarray(eamp, f32).copy()
         else:
-            return self._fail("number of 1-sigma errors in parameter eamp (%d) differs from number of "
-                              "visibility amplitudes in self.amp (%d)." % (len(eamp), len(self.amp)))
+            return self._fail("eamp has %d elements for %d visibility amplitudes" % (len(eamp), len(self.amp)))
         self.base = np.sqrt(self.ulam ** 2 + self.vlam ** 2)
         self.cphs = opt(cphs, f32)
         self.cpix = opt(cpix, np.int32)
@@ -252,18 +251,17 @@ class Mcgpu_Sgra(Mcgpu):
         if self.use_cphs:
             self.ecph = np.ones(ncph, f32)
             if ecph is None:
-                return self._fail("Closure phase 1-sigma error(s) not provided in parameter ecph.")
+                return self._fail("ecph (the closure phases' 1-sigma errors) is required with closure phases")
             elif np.isscalar(ecph):
                 if ecph == 0:
-                    return self._fail("Closure phase 1-sigma error ecph is zero.")
+                    return self._fail("ecph (the closure phases' 1-sigma error) is zero")
                 self.ecph[:] = ecph
             elif len(ecph) == len(self.ecph):
                 if np.any(np.asarray(ecph) == 0):
-                    return self._fail("Some of ecph elements are zero.")
+                    return self._fail("ecph has zero elements")
                 self.ecph = np.asarray(ecph, f32).copy()
             else:
-                return self._fail("number of 1-sigma errors in parameter ecph (%d) differs from number of "
-                                  "closure phaese in self.cphs (%d)." % (len(ecph), len(self.cphs)))
+                return self._fail("ecph has %d elements for %d closure phases" % (len(ecph), len(self.cphs)))
         else:
             self.ecph = None
         # packing (imgpu.py:756-831)
@@ -317,7 +315,7 @@ class Mcgpu_Sgra(Mcgpu):
 
     def burnin_and_search(self):
         if self.error != "":
-            raise ValueError("Cannot burnin and search: first fix the error: " + self.error)
+            raise ValueError("burnin_and_search: the set-up failed: " + self.error)
         Mcgpu.burnin_and_search(self)
         self.mvicp = self.datm[:self.nchi2m].reshape((self.nbeta, self.nseq, self.ndat))
         self.mpha = self.datm[self.nchi2m:].reshape((self.nbeta, self.nseq, self.nvis))
@@ -328,7 +326,7 @@ class Mcgpu_Sgra(Mcgpu):
 
         Returns chi2c, mvicp, mpha, chi2m, datm."""
         if self.error != "":
-            raise ValueError("First fix the error: " + self.error)
+            raise ValueError("the set-up failed: " + self.error)
         prm = np.array(prm_in, dtype=np.float32)
         sh = prm.shape
         if len(sh) == 1:
