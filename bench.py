@@ -170,7 +170,7 @@ def batched_vs_plain(iters, nfits, log_every=16, session_only=False, streams=2):
     tp = 0.0
     plain_vis = 0.0
     for f, (d, nb, ns) in enumerate(fits):
-        with Session([mk(d, nb, ns, f)]) as s:
+        with Session([mk(d, nb, ns, f)], streams=streams) as s:
             s.run(1)
             s.sync()
             e0 = s.stats()
@@ -188,7 +188,7 @@ def batched_vs_plain(iters, nfits, log_every=16, session_only=False, streams=2):
     for f, (d, nb, ns) in enumerate(fits):
         if f % log_every == 0:
             log("  model.cuh path: fit %d/%d" % (f, nfits))
-        with Session([mk(d, nb, ns, f)], force_generic=True) as s:
+        with Session([mk(d, nb, ns, f)], force_generic=True, streams=streams) as s:
             s.run(1)
             s.sync()
             e0 = s.stats()
@@ -208,7 +208,7 @@ def batched_vs_plain(iters, nfits, log_every=16, session_only=False, streams=2):
                 pct_fp32_peak_batched=100.0 * batched * F_VIS / (PEAK_FP32_TFLOPS * 1e12))
 
 
-def adjoint_leg(npts=4096, nvis=20000, ncph=6000, reps=5, gn_fits=64, gn_burn=1000, gn_iter=250):
+def adjoint_leg(npts=4096, nvis=20000, ncph=6000, reps=5, gn_fits=64, gn_burn=1000, gn_iter=250, streams=2):
     """chi^2 + gradient of 4096 points against 20000 visibilities + 6000
     closures, then Gauss-Newton polish + Laplace covariance of 64 fits."""
     import torch
@@ -243,7 +243,7 @@ def adjoint_leg(npts=4096, nvis=20000, ncph=6000, reps=5, gn_fits=64, gn_burn=10
     dsets = [synthdata.ring_gauss_dataset(nvis=2000, ncph=600, nsta=8, seed=500 + f) for f in range(gn_fits)]
     fits = [eht_fit(dd, 32, 128, nburn=gn_burn, niter=gn_iter, seed=500 + f) for f, dd in enumerate(dsets)]
     t0 = time.time()
-    with Session(fits) as s:
+    with Session(fits, streams=streams) as s:
         s.run(gn_burn + gn_iter)
         s.download()
     t_sample = time.time() - t0
@@ -271,7 +271,7 @@ def adjoint_leg(npts=4096, nvis=20000, ncph=6000, reps=5, gn_fits=64, gn_burn=10
     return out
 
 
-def quadsys_leg(nsys=1024, nburn=3000, niter=10000):
+def quadsys_leg(nsys=1024, nburn=3000, niter=10000, streams=2):
     """1024 systems of 8 quadratic equations, every system one fit of one
     session, then Gauss-Newton polish."""
     from cuda_mcmc_amd import quadsys, synthdata
@@ -280,7 +280,7 @@ def quadsys_leg(nsys=1024, nburn=3000, niter=10000):
     fits = [quadsys.quad_fit(ds["Q"][s], ds["b"][s], ds["c"][s], nbeta=16, nseq=32, nburn=nburn, niter=niter,
                              seed=7000 + s, record=False) for s in range(nsys)]
     log("  %d systems packed; sampling" % nsys)
-    with Session(fits) as s:
+    with Session(fits, streams=streams) as s:
         s.run(1)
         s.sync()
         e0 = s.stats()
@@ -332,7 +332,7 @@ def main():
     ap.add_argument("--no-quadsys", action="store_true")
     ap.add_argument("--no-pcie", action="store_true")
     ap.add_argument("--streams", type=int, default=2,
-                    help="sequence groups of the headline and batched sessions")
+                    help="sequence groups of every session (1 for hardware-counter runs)")
     ap.add_argument("--launch-probe", type=int, default=20,
                     help="iterations of the per-launch timing probe (0: no probe session)")
     args = ap.parse_args()
@@ -466,10 +466,10 @@ def main():
                                           session_only=args.batched_session_only, streams=args.streams)
     if not args.no_adjoint:
         log("adjoint gradient + Gauss-Newton...")
-        out["adjoint"] = adjoint_leg()
+        out["adjoint"] = adjoint_leg(streams=args.streams)
     if not args.no_quadsys:
         log("quadratic systems (1024 x 8 equations)...")
-        out["quadsys"] = quadsys_leg()
+        out["quadsys"] = quadsys_leg(streams=args.streams)
     if not args.no_cpu_baseline:
         nth = args.cpu_threads or host_threads()
         log("cpu baseline on %d threads..." % nth)
