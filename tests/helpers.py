@@ -49,3 +49,20 @@ def tie_tol(beta, c_new, c_cur, rel=2e-6):
     outside this band is a real discrepancy and fails the test.
     """
     return 0.5 * beta * rel * (np.abs(c_new) + np.abs(c_cur)) + 1e-6
+
+
+def rosenbrock_moments():
+    """First and second moments of p(x, y) ~ exp(-f^2 / 2) in the box
+    [-5, 5]^2, f = 100 (y - x^2)^2 + (1 - x)^2, by quadrature on a grid
+    (converged to about 1e-4)."""
+    # the density lives along y = x^2 for x in about [-0.8, 2.8]; the prior
+    # box cuts it at y = 5
+    x = np.linspace(-1.5, 3.0, 1801)
+    y = np.linspace(-1.0, 5.0, 4801)
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    f = 100.0 * (Y - X * X) ** 2 + (1.0 - X) ** 2
+    w = np.exp(-0.5 * f * f)
+    z = w.sum()
+    m = {"x": (w * X).sum() / z, "y": (w * Y).sum() / z}
+    m.update(xx=(w * X * X).sum() / z, xy=(w * X * Y).sum() / z, yy=(w * Y * Y).sum() / z)
+    return m

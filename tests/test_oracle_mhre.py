@@ -112,3 +112,24 @@ def test_exchange_always_swaps_equal_chi2():
     np.testing.assert_array_equal(st.tcur, lab)
     assert st.n_exch.sum() == 4 * 5
     np.testing.assert_array_equal(st.tout[:, :, 0], st.tcur)
+
+
+def test_rosenbrock_posterior_matches_quadrature():
+    # the Rosenbrock problem of the GPU test (32 temperatures T = 1 .. 1e3,
+    # 64 sequences, step 0.5 adapted toward 0.25, Philox seed 0), with 2000
+    # sampling iterations instead of 5000 to keep the suite short: the
+    # beta = 1 samples' moments against quadrature of exp(-f^2 / 2)
+    from tests.helpers import rosenbrock_moments
+    f = lambda P: och.generic_chi2("rosenbrock", P, None, [0.0], [1.0])
+    B = 32
+    beta = (10.0 ** (-3.0 * np.arange(B) / (B - 1))).astype(np.float32)
+    pb, st = _run(f, [1, 1], [-5, -5], [5, 5], beta, nseq=64, nburn=2000, niter=2000, seed=0, pstp=0.5,
+                  adapt_target=0.25)
+    xs = st.pout[0, 0].astype(np.float64)
+    ys = st.pout[1, 0].astype(np.float64)
+    per = {"x": xs.mean(axis=1), "y": ys.mean(axis=1), "xx": (xs * xs).mean(axis=1),
+           "xy": (xs * ys).mean(axis=1), "yy": (ys * ys).mean(axis=1)}
+    ref = rosenbrock_moments()
+    for k, v in per.items():
+        se = v.std(ddof=1) / np.sqrt(v.size)
+        assert abs(v.mean() - ref[k]) < 5 * se + 2e-3, (k, v.mean(), ref[k], se)
