@@ -15,10 +15,12 @@ Also reported:
   roofline      the algorithmic FP32 rate of the timed iterations (170 flop
                 per visibility-eval, 15 per closure term, 40 per chain setup)
                 over their device time (HIP events around the window, which
-                hold the step launches of both sequence groups, overlapping
-                on two streams, and the exchange launches), against the
+                hold the step launches of the three sequence groups,
+                overlapping on three streams; each step launch also runs the
+                replica exchange of the iteration before), against the
                 157.3 TFLOP/s peak; also the average duration of one step
-                launch and one exchange launch (HIP events around each);
+                launch and of the exchange launches per iteration (HIP events
+                around each);
   cpu_baseline  the float64 C/OpenMP oracle port on the host cores;
   pcie_inclusive the same whole fit through host buffers (upload, run,
                 download of the recorded samples), host wall time;
@@ -86,9 +88,14 @@ def traffic_from_profiles():
 
 def headline_options(args):
     """Launch form of the plain fit's session: 4 waves per work-group, one
-    wave per chain, the 128 sequences in two groups on two streams
+    wave per chain, the 128 sequences in three groups on three streams
     (--streams changes the group count, e.g. 1 for hardware-counter runs)."""
-    return dict(waves_per_block=4, persistent=0, streams=args.streams)
+    return dict(waves_per_block=4, persistent=0, streams=args.streams or 3)
+
+
+def leg_streams(args):
+    """Sequence groups of the other legs' sessions (two unless --streams)."""
+    return args.streams or 2
 
 
 def host_threads():
@@ -331,8 +338,9 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-quadsys", action="store_true")
     ap.add_argument("--no-pcie", action="store_true")
-    ap.add_argument("--streams", type=int, default=2,
-                    help="sequence groups of every session (1 for hardware-counter runs)")
+    ap.add_argument("--streams", type=int, default=0,
+                    help="sequence groups of every session (0: three for the plain fit, two for the "
+                         "other legs; 1 for hardware-counter runs)")
     ap.add_argument("--launch-probe", type=int, default=20,
                     help="iterations of the per-launch timing probe (0: no probe session)")
     args = ap.parse_args()
@@ -438,7 +446,7 @@ def main():
                      "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                      "traffic": traffic_from_profiles(),
                      "kernel": "one MH-RE iteration: k_step_eht<Sgra9> on each of %d sequence-group streams "
-                               "(overlapping) + k_exchange" % ngroups,
+                               "(overlapping), each also running the exchange of the iteration before" % ngroups,
                      "iteration_ms": round(gpu_ms / max(steps, 1), 5),
                      "step_launches_per_iteration": ngroups,
                      "step_launch_ms": None if step_ms is None else round(step_ms, 5),
@@ -463,13 +471,13 @@ def main():
     if not args.no_batched:
         log("batched vs plain (%d fits)..." % args.batched_fits)
         out["batched"] = batched_vs_plain(args.batched_iters, args.batched_fits,
-                                          session_only=args.batched_session_only, streams=args.streams)
+                                          session_only=args.batched_session_only, streams=leg_streams(args))
     if not args.no_adjoint:
         log("adjoint gradient + Gauss-Newton...")
-        out["adjoint"] = adjoint_leg(streams=args.streams)
+        out["adjoint"] = adjoint_leg(streams=leg_streams(args))
     if not args.no_quadsys:
         log("quadratic systems (1024 x 8 equations)...")
-        out["quadsys"] = quadsys_leg(streams=args.streams)
+        out["quadsys"] = quadsys_leg(streams=leg_streams(args))
     if not args.no_cpu_baseline:
         nth = args.cpu_threads or host_threads()
         log("cpu baseline on %d threads..." % nth)

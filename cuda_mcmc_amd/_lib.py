@@ -58,6 +58,7 @@ class MhreOptions(C.Structure):
         ("waves_per_block", C.c_int), ("persistent", C.c_int), ("graph_iters", C.c_int),
         ("trace_iters", C.c_int), ("trace_dec", ip), ("trace_chi2", fp), ("trace_ex", ip),
         ("force_generic", C.c_int), ("waves_per_simd", C.c_int), ("streams", C.c_int),
+        ("exchange_launch", C.c_int),
     ]
 
 

@@ -112,6 +112,8 @@ struct FitHost {
   size_t o_coor, o_dat, o_std2r, o_icoor, o_idat, o_pdescr, o_ivar, o_invar, o_pmint, o_pmaxt, o_beta;
   size_t o_pcur, o_chi2c, o_pstp, o_ncnt, o_nacpt, o_nhist, o_nexch, o_sumh, o_tcur, o_flag, o_ptent, o_ptentn;
   size_t o_pout, o_tout, o_chi2, o_datm, o_chi2m, o_mc;
+  size_t o_pcur_b = 0, o_chi2c_b = 0, o_tcur_b = 0;   // second state set (xfused)
+  bool xfused = false;           // exchange inside the next step launch (FitDesc::xfused)
   bool has_datm = false;
   bool record = true;            // pout/tout/chi2 given
   FitDesc d{};
@@ -343,6 +345,13 @@ int hist_words(int nadj) { return (nadj + 31) / 32; }
 // one launch (k_iter_quad).
 bool quad_fused(const FitHost& h) { return h.kind == K_QUAD && h.s.nbeta <= 64; }
 
+// Ring+Gaussian fits whose ladder fits one wave exchange at the start of the
+// next step launch (FitDesc::xfused) unless the caller asks for the exchange
+// launch of every iteration.
+bool eht_xfused(const FitHost& h, const MhreOptions& o) {
+  return (h.kind == K_SGRA9 || h.kind == K_SGRA13) && h.s.nbeta <= 64 && o.exchange_launch == 0;
+}
+
 // Most concurrent sequence groups (streams) of a session.
 constexpr int MAX_GROUPS = 8;
 
@@ -375,7 +384,7 @@ void unpack_hist(const std::vector<uint32_t>& bits, int nadj, size_t npc, int* n
 struct KindWork {
   int kind;
   int quad_n;
-  bool fused = false;   // quadratic systems: step + exchange launch, items (fit, first sequence)
+  bool fused = false;   // quadratic systems: step and exchange in one launch, items (fit, first sequence)
   std::vector<WorkItem> step, eval;
   XcdRanges xr{};   // wave-per-chain kinds: step list dealt into per-XCD ranges
   WorkItem* d_step = nullptr;
@@ -393,9 +402,11 @@ struct Group {
   hipStream_t stream = nullptr;
   RunClock* d_clk = nullptr;
   std::vector<KindWork> kinds;   // step lists of the group's chains
-  WorkItem* d_x = nullptr;       // exchange list of the group's sequences
+  WorkItem* d_x = nullptr;       // sequences exchanged by a launch of their own every iteration
   int n_x = 0;
   XcdRanges xr_x{};
+  WorkItem* d_xc = nullptr;      // xfused sequences, finished by a segment's closing launch
+  XcdRanges xr_xc{};
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   hipEvent_t done = nullptr;
@@ -596,13 +607,14 @@ struct Downloads {
   }
 };
 
-StepLaunch make_launch(Session* S, const Group& g, const KindWork& k) {
+StepLaunch make_launch(Session* S, const Group& g, const KindWork& k, int it_off) {
   StepLaunch L{};
   L.fits = S->d_fits;
   L.work = k.d_step;
   L.nwork = (int)k.step.size();
   L.xr = k.xr;
   L.clk = g.d_clk;
+  L.it_off = it_off;
   L.wpb = S->wpb;
   L.persistent = S->persistent;
   L.grid = S->grid;
@@ -617,20 +629,23 @@ StepLaunch make_launch(Session* S, const Group& g, const KindWork& k) {
   return L;
 }
 
-// The exchange launch of a group's iteration (with the decision trace, if any).
-StepLaunch exchange_launch(Session* S, const Group& g) {
+// An exchange launch of group g (with the decision trace, if any).
+StepLaunch exchange_launch(Session* S, const Group& g, int it_off, int close_k) {
   StepLaunch L{};
   L.fits = S->d_fits;
   L.clk = g.d_clk;
+  L.it_off = it_off;
+  L.close_k = close_k;
   L.tr_ex = S->d_tr_ex;
   L.tr_it0 = S->tr_it0;
   L.tr_nit = S->tr_nit;
   return L;
 }
 
-int launch_steps(Session* S, const Group& g) {
+// The step launches of group g at offset i of its segment.
+int launch_steps(Session* S, const Group& g, int i) {
   for (const KindWork& k : g.kinds) {
-    const StepLaunch L = make_launch(S, g, k);
+    const StepLaunch L = make_launch(S, g, k, i);
     hipError_t e = hipSuccess;
     if (k.kind == K_SGRA9) e = launch_step_eht(MODEL_SGRA9, L, g.stream);
     else if (k.kind == K_SGRA13) e = launch_step_eht(MODEL_SGRA13, L, g.stream);
@@ -642,17 +657,33 @@ int launch_steps(Session* S, const Group& g) {
   return MHRE_OK;
 }
 
-// The exchange launch of group g (with no sequence to exchange, it only
-// hands the clock to the next iteration).
-int launch_group_exchange(Session* S, const Group& g) {
-  HIPCHK(launch_exchange(exchange_launch(S, g), g.d_x, g.xr_x, g.stream));
+// The exchange launch at offset i of group g's segment, for the sequences
+// that exchange in a launch of their own every iteration (none: no launch).
+int launch_group_exchange(Session* S, const Group& g, int i) {
+  if (g.n_x > 0) HIPCHK(launch_exchange(exchange_launch(S, g, i, 0), g.d_x, g.xr_x, g.stream));
   return MHRE_OK;
 }
 
-// One iteration of group g on its stream: the step launches, then the exchange.
-int launch_iteration(Session* S, const Group& g) {
-  CHK(launch_steps(S, g));
-  return launch_group_exchange(S, g);
+// Offset i of group g's segment on its stream: the step launches, then the
+// exchange launch of the sequences that need one.
+int launch_iteration(Session* S, const Group& g, int i) {
+  CHK(launch_steps(S, g, i));
+  return launch_group_exchange(S, g, i);
+}
+
+// The closing launch of group g's segment of K iterations: the xfused
+// sequences' last exchange, then the clock moves on by K.
+int close_segment(Session* S, const Group& g, int K) {
+  HIPCHK(launch_exchange(exchange_launch(S, g, 0, K), g.d_xc, g.xr_xc, g.stream));
+  return MHRE_OK;
+}
+
+// A segment of K iterations of every group, the groups' launches interleaved.
+int launch_segment(Session* S, int K) {
+  for (int i = 0; i < K; ++i)
+    for (const Group& g : S->groups) CHK(launch_iteration(S, g, i));
+  for (const Group& g : S->groups) CHK(close_segment(S, g, K));
+  return MHRE_OK;
 }
 
 // The other groups' streams start after what the session stream holds so far
@@ -671,21 +702,23 @@ int join_groups(Session* S) {
   return MHRE_OK;
 }
 
-// Iterations with HIP events around every launch (on each group's stream):
-// average device time per launch of the step kernels and of the exchange,
-// over groups and iterations (launches of different groups overlap).  All
-// launches and events of a batch are queued before the host waits, so the
-// gaps between launches are not counted.
+// Iterations (in segments of BATCH) with HIP events around every launch, on
+// each group's stream: average device time per iteration of a group's step
+// launches and of its exchange launches (the per-iteration ones plus the
+// closing launch of each segment), over groups and iterations (launches of
+// different groups overlap).  All launches and events of a segment are queued
+// before the host waits, so the gaps between launches are not counted.
 int time_kernels(Session* S, int niter, double* step_ms, double* exch_ms) {
   (void)hipGetLastError();
   constexpr int BATCH = 32;
   const int G = (int)S->groups.size();
-  std::vector<hipEvent_t> ev((size_t)3 * BATCH * G, nullptr);
+  std::vector<hipEvent_t> ev((size_t)3 * BATCH * G + 2 * G, nullptr);
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   struct Destroy {
     std::vector<hipEvent_t>& v;
     ~Destroy() { for (auto e : v) if (e) hipEventDestroy(e); }
   } destroy{ev};
+  hipEvent_t* ec = &ev[(size_t)3 * BATCH * G];   // closing launches: 2 per group
   double st = 0.0, ex = 0.0;
   HIPCHK(hipEventRecord(S->ev0, S->stream));
   CHK(fork_groups(S, S->ev0));
@@ -696,19 +729,30 @@ int time_kernels(Session* S, int niter, double* step_ms, double* exch_ms) {
         const Group& g = S->groups[gi];
         hipEvent_t* e = &ev[(size_t)3 * (i * G + gi)];
         HIPCHK(hipEventRecord(e[0], g.stream));
-        CHK(launch_steps(S, g));
+        CHK(launch_steps(S, g, i));
         HIPCHK(hipEventRecord(e[1], g.stream));
-        CHK(launch_group_exchange(S, g));
+        CHK(launch_group_exchange(S, g, i));
         HIPCHK(hipEventRecord(e[2], g.stream));
       }
     }
-    for (int gi = 0; gi < G; ++gi) HIPCHK(hipEventSynchronize(ev[(size_t)3 * ((nb - 1) * G + gi) + 2]));
+    for (int gi = 0; gi < G; ++gi) {
+      const Group& g = S->groups[gi];
+      HIPCHK(hipEventRecord(ec[2 * gi], g.stream));
+      CHK(close_segment(S, g, nb));
+      HIPCHK(hipEventRecord(ec[2 * gi + 1], g.stream));
+    }
+    for (int gi = 0; gi < G; ++gi) HIPCHK(hipEventSynchronize(ec[2 * gi + 1]));
     for (int i = 0; i < nb * G; ++i) {
       float a = 0.0f, b = 0.0f;
       HIPCHK(hipEventElapsedTime(&a, ev[(size_t)3 * i], ev[(size_t)3 * i + 1]));
       HIPCHK(hipEventElapsedTime(&b, ev[(size_t)3 * i + 1], ev[(size_t)3 * i + 2]));
       st += a;
       ex += b;
+    }
+    for (int gi = 0; gi < G; ++gi) {
+      float c = 0.0f;
+      HIPCHK(hipEventElapsedTime(&c, ec[2 * gi], ec[2 * gi + 1]));
+      ex += c;
     }
     done += nb;
     S->iters_done += nb;
@@ -734,8 +778,7 @@ void drop_graph(Session* S) {
 int time_iterations(Session* S, int niter, double* ms) {
   HIPCHK(hipEventRecord(S->ev0, S->stream));
   CHK(fork_groups(S, S->ev0));
-  for (int i = 0; i < niter; ++i)
-    for (const Group& g : S->groups) CHK(launch_iteration(S, g));
+  if (niter > 0) CHK(launch_segment(S, niter));
   CHK(join_groups(S));
   HIPCHK(hipEventRecord(S->ev1, S->stream));
   HIPCHK(hipEventSynchronize(S->ev1));
@@ -859,6 +902,10 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
     h.o_nexch = A.reserve<int>(C);
     h.o_sumh = A.reserve<int>(P * C);
     h.o_tcur = A.reserve<int>(C);
+    h.xfused = eht_xfused(h, S->opt);
+    h.o_pcur_b = A.reserve<float>(h.xfused ? C * N : 0);
+    h.o_chi2c_b = A.reserve<float>(h.xfused ? C : 0);
+    h.o_tcur_b = A.reserve<int>(h.xfused ? C : 0);
     h.o_flag = A.reserve<int>(C);
     h.o_ptent = A.reserve<float>(C);
     h.o_ptentn = A.reserve<int>(C);
@@ -942,6 +989,10 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
     d.xlanes = exchange_lanes(s.nbeta);
     d.n_exch = at<int>(S, h.o_nexch); d.sumh = at<int>(S, h.o_sumh); d.tcur = at<int>(S, h.o_tcur);
     d.flag = at<int>(S, h.o_flag); d.ptent = at<float>(S, h.o_ptent); d.ptentn = at<int>(S, h.o_ptentn);
+    d.xfused = h.xfused ? 1 : 0;
+    if (h.xfused) {
+      d.pcur_b = at<float>(S, h.o_pcur_b); d.chi2c_b = at<float>(S, h.o_chi2c_b); d.tcur_b = at<int>(S, h.o_tcur_b);
+    }
     d.pout = at<float>(S, h.o_pout); d.tout = at<int>(S, h.o_tout); d.chi2 = at<float>(S, h.o_chi2);
     d.datm = h.has_datm ? at<float>(S, h.o_datm) : nullptr;
     d.chi2m = h.has_datm ? at<float>(S, h.o_chi2m) : nullptr;
@@ -1041,8 +1092,8 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
   // at multiples of its exchange wave's sequence count (64 / xlanes), so no
   // exchange wave and no chain is shared between groups.
   // At most as many groups as the largest fit has exchange waves, so every
-  // group holds sequences (its exchange launch advances its clock, and the
-  // download reads group 0's clock).
+  // group holds sequences (its step launches hand its clock to its closing
+  // launch, and the download reads group 0's clock).
   int max_waves = 1;
   for (const FitHost& h : S->fits) {
     const int per = 64 / exchange_lanes(h.s.nbeta);
@@ -1128,21 +1179,27 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
       if (kk != K_USER) kw.xr = deal_xcd(kw.step, fit_cost);
       grp.kinds.push_back(std::move(kw));
     }
-    // exchange list, heaviest fits first, dealt like the step lists;
-    // one item per wave: its first sequence (64 / xlanes sequences per wave)
-    std::vector<WorkItem> xw;
+    // exchange lists, heaviest fits first, dealt like the step lists; one
+    // item per wave: its first sequence (64 / xlanes sequences per wave).
+    // xw: sequences exchanged by a launch of their own every iteration; xc:
+    // xfused sequences (their segment's closing launch finishes them).
+    std::vector<WorkItem> xw, xc;
     for (int i : xorder) {
       if (quad_fused(S->fits[i])) continue;   // exchanged inside its step launch
       const int per = 64 / exchange_lanes(S->fits[i].s.nbeta);
       int s0, s1;
       seq_range(i, g, &s0, &s1);
-      for (int q = s0; q < s1; q += per) xw.push_back({i, q});
+      for (int q = s0; q < s1; q += per) (S->fits[i].xfused ? xc : xw).push_back({i, q});
     }
     grp.xr_x = deal_xcd(xw, seq_cost);
     grp.n_x = (int)xw.size();
+    grp.xr_xc = deal_xcd(xc, seq_cost);
     CHK(dev_alloc(S, &grp.d_x, xw.size()));
+    CHK(dev_alloc(S, &grp.d_xc, xc.size()));
     if (!xw.empty())
       HIPCHK(hipMemcpyAsync(grp.d_x, xw.data(), xw.size() * sizeof(WorkItem), hipMemcpyHostToDevice, S->stream));
+    if (!xc.empty())
+      HIPCHK(hipMemcpyAsync(grp.d_xc, xc.data(), xc.size() * sizeof(WorkItem), hipMemcpyHostToDevice, S->stream));
     for (auto& k : grp.kinds) {
       CHK(dev_alloc(S, &k.d_step, k.step.size()));
       HIPCHK(hipMemcpyAsync(k.d_step, k.step.data(), k.step.size() * sizeof(WorkItem), hipMemcpyHostToDevice,
@@ -1233,12 +1290,14 @@ int run(Session* S, int niter) {
   int left = niter;
   const int G = S->opt.graph_iters;
   if (G > 0 && niter >= G) {
-    // one graph of G iterations per group, captured on the group's stream
+    // one graph per group: a segment of G iterations and its closing launch,
+    // captured on the group's stream
     for (Group& grp : S->groups) {
       if (grp.gexec) continue;
       HIPCHK(hipStreamBeginCapture(grp.stream, hipStreamCaptureModeThreadLocal));
       int rc = MHRE_OK;
-      for (int i = 0; i < G && rc == MHRE_OK; ++i) rc = launch_iteration(S, grp);
+      for (int i = 0; i < G && rc == MHRE_OK; ++i) rc = launch_iteration(S, grp, i);
+      if (rc == MHRE_OK) rc = close_segment(S, grp, G);
       hipGraph_t g = nullptr;
       const hipError_t ec = hipStreamEndCapture(grp.stream, &g);
       if (rc != MHRE_OK || ec != hipSuccess) {
@@ -1254,8 +1313,7 @@ int run(Session* S, int niter) {
       left -= G;
     }
   }
-  for (int i = 0; i < left; ++i)
-    for (const Group& grp : S->groups) CHK(launch_iteration(S, grp));
+  if (left > 0) CHK(launch_segment(S, left));
   CHK(join_groups(S));
   HIPCHK(hipEventRecord(S->ev1, S->stream));
   S->iters_done += niter;

@@ -15,16 +15,17 @@ __device__ inline const FitDesc& wave_fit(const FitDesc* fits, int fit) {
   return *(const FitDesc*)wave_const(fits + __builtin_amdgcn_readfirstlane(fit));
 }
 
-// Step launches: the first thread of the grid hands the iteration on to the
-// exchange launch that follows (RunClock); every thread reads it with
-// load_clock.  Kept as two calls so that a kernel can do the hand-over
+// Step launches: the first thread of the grid hands the segment's first
+// iteration on to the segment's closing exchange launch (RunClock); every
+// thread reads its iteration with load_clock (clk.it plus the launch's offset
+// in the segment).  Kept as two calls so that a kernel can do the hand-over
 // before its set-up (which, in the fused step kernel, keeps its register
-// allocation at 96 VGPRs).
+// allocation down).
 __device__ inline void pass_clock(RunClock* clk) {
   if (blockIdx.x == 0 && threadIdx.x == 0) clk->xit = clk->it;
 }
-__device__ inline int load_clock(const RunClock* clk) {
-  return __atomic_load_n(&clk->it, __ATOMIC_RELAXED);
+__device__ inline int load_clock(const RunClock* clk, int it_off) {
+  return __atomic_load_n(&clk->it, __ATOMIC_RELAXED) + it_off;
 }
 
 // Adds the wave's performed evaluations and proposals to the fit's counters
@@ -56,6 +57,9 @@ struct StepLaunch {
   int nwork;
   XcdRanges xr;          // wave-per-chain kernels: per-XCD ranges of work
   RunClock* clk;         // device
+  int it_off;            // the launch's offset in its segment: iteration clk.it + it_off
+  int close_k;           // exchange launch: 0 = the exchange of iteration clk.it + it_off (fits
+                         // without xfused), K > 0 = closing launch of a segment of K iterations
   int wpb;               // waves per work-group (1, 2, 4)
   int persistent;        // 1: grid-stride work queue
   int grid;              // work-groups for the persistent form
@@ -74,6 +78,10 @@ hipError_t launch_step_quad(int n, const StepLaunch& L, hipStream_t s);
 // step + exchange of whole sequences (ladders of at most 64 rungs); work items
 // are (fit, first sequence of the wave)
 hipError_t launch_iter_quad(int n, const StepLaunch& L, hipStream_t s);
+// Exchange of the sequences in xwork (items: fit, first sequence of the wave):
+// per iteration for fits without xfused (L.close_k = 0); the closing launch of
+// a segment (L.close_k = K) finishes the xfused fits' last exchange, leaves
+// their state in the first set of arrays and moves the clock on by K.
 hipError_t launch_exchange(const StepLaunch& L, const WorkItem* xwork, const XcdRanges& xr, hipStream_t s);
 
 // generic.hip: the library's compiled-in model.cuh (reference contract)

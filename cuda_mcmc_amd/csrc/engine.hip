@@ -311,16 +311,125 @@ __global__ void __launch_bounds__(256) k_eval_eht(const FitDesc* fits, const Wor
 
 // ------------------------------------------------------------ step kernel
 
+// The replica exchange of iteration itp of sequence iseq (reference:
+// gpu_mcmc.cu:548-624, the same draws and arithmetic as exchange_wave),
+// computed inside one wave whose lane j holds temperature slot j (B <= 64)
+// with chi^2 and labels read from c2s and tcs.  Every wave of the sequence
+// computes it and keeps what its own slot sl needs: the slot whose row ends
+// up in sl, and sl's chi^2 and label after the swaps.  It also does the
+// exchange launch's per-slot writes for sl (n_exch) and, in slot 0's wave,
+// the sequence's trace entries.
+struct SlotAfterSwaps {
+  int src;
+  float c2;
+  int tc;
+};
+__device__ inline SlotAfterSwaps seq_exchange(const FitDesc& f, int iseq, int sl, int itp, const float* c2s,
+                                              const int* tcs, const TraceBuf& tr) {
+  const int lane = lane_id();
+  const int B = f.nbeta, S = f.nseq;
+  const bool burn = itp < f.nburn;
+  const bool slot = lane < B;
+  float c2 = slot ? c2s[(size_t)lane * S + iseq] : 0.0f;
+  int tc = slot ? tcs[(size_t)lane * S + iseq] : 0;
+  int perm = lane;
+  int nex = 0;
+  if (B > 1) {
+    // attempt a's pair pick and log u on lane a, all drawn up front
+    int k_l = 0;
+    double lu_l = 0.0, hb_l = 0.0;
+    if (slot) {
+      const U4 r = philox4x32_10((uint32_t)iseq, (uint32_t)itp, (uint32_t)lane, RNG_EXCH, f.key0, f.key1);
+      k_l = pick_with_retry(r.x, B - 1, (uint32_t)iseq, (uint32_t)itp, (uint32_t)lane, RNG_EXCH, f.key0, f.key1);
+      lu_l = log((double)u01(r.y));
+    }
+    if (lane < B - 1) hb_l = -0.5 * ((double)f.beta[lane + 1] - (double)f.beta[lane]);
+    const bool trace = tr.ex && sl == 0 && itp >= tr.it0 && itp < tr.it0 + tr.nit;
+    for (int a = 0; a < B; ++a) {   // every value below is uniform: lane reads, no shuffles
+      const int k = bcast(k_l, a);
+      const float ck = bcast(c2, k), ck1 = bcast(c2, k + 1);
+      const double x = bcast(hb_l, k) * ((double)ck - (double)ck1);
+      const bool acc = bcast(lu_l, a) <= x;
+      if (acc) {
+        const int tk = bcast(tc, k), tk1 = bcast(tc, k + 1);
+        const int pk = bcast(perm, k), pk1 = bcast(perm, k + 1);
+        if (lane == k) { c2 = ck1; tc = tk1; perm = pk1; if (!burn) ++nex; }
+        if (lane == k + 1) { c2 = ck; tc = tk; perm = pk; }
+      }
+      if (trace && lane == 0)
+        tr.ex[(size_t)f.tr_ex + ((size_t)(itp - tr.it0) * B + a) * S + iseq] = k | ((acc ? 1 : 0) << 16);
+    }
+  }
+  const int nx = bcast(nex, sl);
+  if (!burn && nx && lane == 0) f.n_exch[(size_t)sl * S + iseq] += nx;
+  return SlotAfterSwaps{bcast(perm, sl), bcast(c2, sl), bcast(tc, sl)};
+}
+
+// One MH iteration of chain ch (all nprm single-parameter proposals) by one
+// wave.  With f.xfused the launch at offset it_off > 0 of its segment first
+// runs the exchange of iteration it - 1 (the state arrays alternate between
+// the two sets by the parity of it_off); for it past the fit's last iteration
+// only that exchange runs.
 template <class M>
-__device__ void step_one_chain(const FitDesc& f, int ch, int it, float2* ring, const BesselTab& bt,
+__device__ void step_one_chain(const FitDesc& f, int ch, int it, int it_off, float2* ring, const BesselTab& bt,
                                const TraceBuf& tr) {
   const int lane = lane_id();
   const int C = f.nchain, P = f.nprm, N = f.nptot;
   const int ibeta = ch / f.nseq;
   const bool burn = it < f.nburn;
+  const int nit = f.nburn + f.niter;
+
+  // the chain's state before the step (after the previous exchange), and
+  // where the state after the step goes
+  float* pdst = f.pcur;
+  float* cdst = f.chi2c;
+  int* tdst = nullptr;
+  float pl;
+  float chi2c;
+  int tc = 0;
+  if (f.xfused) {
+    const bool odd = (it_off & 1) != 0;
+    if (odd) { pdst = f.pcur_b; cdst = f.chi2c_b; }
+    tdst = odd ? f.tcur_b : f.tcur;
+    if (it_off > 0) {
+      const int iseq = ch - ibeta * f.nseq;
+      const float* ps = odd ? f.pcur : f.pcur_b;
+      const SlotAfterSwaps x = seq_exchange(f, iseq, ibeta, it - 1, odd ? f.chi2c : f.chi2c_b,
+                                            odd ? f.tcur : f.tcur_b, tr);
+      pl = lane < N ? ps[((size_t)x.src * f.nseq + iseq) * N + lane] : 0.0f;
+      chi2c = x.c2;
+      tc = x.tc;
+      if (it - 1 >= f.nburn && f.record) {
+        // the sample of iteration it - 1 in slot ibeta (what the exchange launch records)
+        const int itr = it - 1 - f.nburn;
+        if (lane < N) {
+          const int ip = f.invar[lane];
+          if (ip >= 0) f.pout[(((size_t)f.nbeta * ip + ibeta) * f.nseq + iseq) * f.niter + itr] = pl;
+        }
+        if (lane == 0) {
+          f.tout[(size_t)ch * f.niter + itr] = tc;
+          if (ibeta == 0) f.chi2[(size_t)iseq * f.niter + itr] = chi2c;
+        }
+      }
+      if (it >= nit) {   // the fit's last exchange: no step
+        if (lane < N) pdst[(size_t)ch * N + lane] = pl;
+        if (lane == 0) {
+          cdst[ch] = chi2c;
+          tdst[ch] = tc;
+        }
+        return;
+      }
+    } else {
+      pl = lane < N ? f.pcur[(size_t)ch * N + lane] : 0.0f;
+      chi2c = f.chi2c[ch];
+      tc = f.tcur[ch];
+    }
+  } else {
+    pl = lane < N ? f.pcur[(size_t)ch * N + lane] : 0.0f;
+    chi2c = f.chi2c[ch];
+  }
 
   // lane-distributed chain state and constants
-  float pl = lane < N ? f.pcur[(size_t)ch * N + lane] : 0.0f;
   const int pdl = lane < N ? f.pdescr[lane] : 0;
   const float pminl = lane < N ? f.pmint[lane] : 0.0f;
   const float pmaxl = lane < N ? f.pmaxt[lane] : 0.0f;
@@ -337,7 +446,6 @@ __device__ void step_one_chain(const FitDesc& f, int ch, int it, float2* ring, c
     hword = f.hbits[((size_t)hw_idx * P + lane) * C + ch];
   }
   const double beta = (double)f.beta[ibeta];
-  float chi2c = f.chi2c[ch];
   int flag = 0, tlast = 0;
   float told = 0.0f;
   unsigned nev = 0;
@@ -418,7 +526,7 @@ __device__ void step_one_chain(const FitDesc& f, int ch, int it, float2* ring, c
     }
   }
 
-  if (lane < N) f.pcur[(size_t)ch * N + lane] = pl;
+  if (lane < N) pdst[(size_t)ch * N + lane] = pl;
   if (lane < P) {
     f.pstp[(size_t)lane * C + ch] = stp;
     f.n_cnt[(size_t)lane * C + ch] = cnt;
@@ -429,7 +537,8 @@ __device__ void step_one_chain(const FitDesc& f, int ch, int it, float2* ring, c
     }
   }
   if (lane == 0) {
-    f.chi2c[ch] = chi2c;
+    cdst[ch] = chi2c;
+    if (tdst) tdst[ch] = tc;
     f.flag[ch] = flag;
     f.ptent[ch] = told;
     f.ptentn[ch] = tlast;
@@ -440,46 +549,58 @@ __device__ void step_one_chain(const FitDesc& f, int ch, int it, float2* ring, c
   }
 }
 
-// Next work item of a persistent step kernel's queue q: one atomic per wave
-// (leader lane), broadcast to the wave.
-__device__ inline int grab_item(RunClock* clk, int q) {
+// Next work item of a persistent step kernel's queue q of head set hs: one
+// atomic per wave (leader lane), broadcast to the wave.
+__device__ inline int grab_item(RunClock* clk, int hs, int q) {
   const int leader = __builtin_amdgcn_readfirstlane(lane_id());
   int v = 0;
-  if (lane_id() == leader) v = atomicAdd(&clk->qhead[q], 1);
+  if (lane_id() == leader) v = atomicAdd(&clk->qhead[hs][q], 1);
   return __builtin_amdgcn_readfirstlane(v);
+}
+
+// The last iteration at which a launch has work for fit f: its last step,
+// or, with f.xfused and a launch that is not the first of its segment, the
+// exchange of that step.
+__device__ inline int last_busy_it(const FitDesc& f, int it_off) {
+  return f.nburn + f.niter - 1 + ((f.xfused && it_off > 0) ? 1 : 0);
 }
 
 // Grid form: work-group b serves the range of XCD b % NXCD.  Persistent form:
 // each wave drains its own XCD's queue, then helps the other ranges in turn;
 // every queue head only grows, so every wave leaves once all are exhausted.
+// The queues use head set it_off & 1; the first thread clears the other set
+// for the next launch of the segment.
 template <class M, int WPB, bool PERSIST>
 __global__ void __launch_bounds__(64 * WPB) k_step_eht(const FitDesc* fits, const WorkItem* work, XcdRanges xr,
-                                                       RunClock* clk, TraceBuf tr, int qslot) {
+                                                       RunClock* clk, TraceBuf tr, int qslot, int it_off) {
   __shared__ float2 ring_all[WPB][RING];
   __shared__ float4 btab[2 * BESSEL_TAB_N];
   pass_clock(clk);
+  if (PERSIST && blockIdx.x == 0 && threadIdx.x < NXCD) clk->qhead[(it_off + 1) & 1][qslot * NXCD + threadIdx.x] = 0;
   load_bessel_tab(btab);
   const BesselTab bt{btab};
   const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   float2* ring = ring_all[wid];
-  const int it = __builtin_amdgcn_readfirstlane(load_clock(clk));
+  const int it = __builtin_amdgcn_readfirstlane(load_clock(clk, it_off));
   const int xb = (int)(blockIdx.x % NXCD);
   if (!PERSIST) {
     const int wv = xr.off[xb] + (int)(blockIdx.x / NXCD) * WPB + wid;
     if (wv >= xr.off[xb + 1]) return;
     const WorkItem w = work[wv];
     const FitDesc& f = wave_fit(fits, w.fit);
-    if (it >= f.nburn + f.niter) return;
-    step_one_chain<M>(f, __builtin_amdgcn_readfirstlane(w.idx), it, ring, bt, tr);
+    if (it > last_busy_it(f, it_off)) return;
+    step_one_chain<M>(f, __builtin_amdgcn_readfirstlane(w.idx), it, it_off, ring, bt, tr);
   } else {
+    const int hs = it_off & 1;
     for (int r = 0; r < NXCD; ++r) {
       const int x = (xb + r) % NXCD;
       const int base = xr.off[x], len = xr.off[x + 1] - base;
       const int q = qslot * NXCD + x;
-      for (int j = grab_item(clk, q); j < len; j = grab_item(clk, q)) {
+      for (int j = grab_item(clk, hs, q); j < len; j = grab_item(clk, hs, q)) {
         const WorkItem w = work[base + j];
         const FitDesc& f = wave_fit(fits, w.fit);
-        if (it < f.nburn + f.niter) step_one_chain<M>(f, __builtin_amdgcn_readfirstlane(w.idx), it, ring, bt, tr);
+        if (it <= last_busy_it(f, it_off))
+          step_one_chain<M>(f, __builtin_amdgcn_readfirstlane(w.idx), it, it_off, ring, bt, tr);
       }
     }
   }
@@ -682,7 +803,7 @@ __device__ unsigned quad_one_chain(const FitDesc& f, int ch, int it, const Trace
 // resident waves hide.
 template <int NQ>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8)))
-k_step_quad(const FitDesc* fits, const WorkItem* work, int nwork, RunClock* clk, TraceBuf tr) {
+k_step_quad(const FitDesc* fits, const WorkItem* work, int nwork, RunClock* clk, TraceBuf tr, int it_off) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (__builtin_amdgcn_readfirstlane(g) >= nwork) return;   // whole waves (the list is padded to 64)
   const WorkItem w = work[g];
@@ -690,7 +811,7 @@ k_step_quad(const FitDesc* fits, const WorkItem* work, int nwork, RunClock* clk,
   // are read through scalar loads
   const FitDesc& f = wave_fit(fits, w.fit);
   pass_clock(clk);
-  const int it = __builtin_amdgcn_readfirstlane(load_clock(clk));
+  const int it = __builtin_amdgcn_readfirstlane(load_clock(clk, it_off));
   if (it >= f.nburn + f.niter) return;
   const bool live = w.idx >= 0;   // padding lanes only take part in the counting
   const unsigned nev = live ? quad_one_chain<NQ>(f, w.idx, it, tr) : 0u;
@@ -703,21 +824,25 @@ k_step_quad(const FitDesc* fits, const WorkItem* work, int nwork, RunClock* clk,
 // lane g*W + j holds temperature slot j of sequence iseq0 + g.  The swap
 // attempts of one sequence are sequential (reference: gpu_mcmc.cu:548-624);
 // the groups of a wave run theirs side by side, exchanging values with lane
-// shuffles inside each group.
-__device__ void exchange_wave(const FitDesc& f, int iseq0, int it, bool burn, const TraceBuf& tr) {
+// shuffles inside each group.  The state is read from (ps, cs, ts) and
+// written to (pd, cd, td), which may be the same arrays; with swaps = false
+// the state is only copied (no attempt, no sample, no trace).
+__device__ void exchange_wave(const FitDesc& f, int iseq0, int it, bool swaps, const float* ps, const float* cs,
+                              const int* ts, float* pd, float* cd, int* td, const TraceBuf& tr) {
   const int lane = lane_id();
   const int B = f.nbeta, S = f.nseq, N = f.nptot, W = f.xlanes;
+  const bool burn = it < f.nburn;
   const int sl = lane & (W - 1);   // temperature slot
   const int gb = lane - sl;        // first lane of this sequence's group
   const int iseq = iseq0 + lane / W;
   const bool seq_ok = iseq < S;
   const bool slot = seq_ok && sl < B;
   const size_t me = (size_t)sl * S + iseq;
-  float c2 = slot ? f.chi2c[me] : 0.0f;
-  int tc = slot ? f.tcur[me] : 0;
+  float c2 = slot ? cs[me] : 0.0f;
+  int tc = slot ? ts[me] : 0;
   int perm = sl;
   int nex = 0;
-  if (B > 1) {
+  if (swaps && B > 1) {
     // attempt a's pair pick and log u on slot a, all drawn up front
     int k_l = 0;
     double lu_l = 0.0, hb_l = 0.0;
@@ -747,18 +872,18 @@ __device__ void exchange_wave(const FitDesc& f, int iseq0, int it, bool burn, co
   float row[MAXP];
 #pragma unroll
   for (int j = 0; j < MAXP; ++j)
-    row[j] = (slot && j < N) ? f.pcur[((size_t)perm * S + iseq) * N + j] : 0.0f;
-  if (slot && perm != sl) {
+    row[j] = (slot && j < N) ? ps[((size_t)perm * S + iseq) * N + j] : 0.0f;
+  if (slot && (perm != sl || pd != ps)) {
 #pragma unroll
     for (int j = 0; j < MAXP; ++j)
-      if (j < N) f.pcur[me * N + j] = row[j];
+      if (j < N) pd[me * N + j] = row[j];
   }
   if (slot) {
-    f.chi2c[me] = c2;
-    f.tcur[me] = tc;
+    cd[me] = c2;
+    td[me] = tc;
     if (!burn && nex) f.n_exch[me] += nex;
   }
-  if (!burn && f.record) {
+  if (swaps && !burn && f.record) {
     const int itr = it - f.nburn;
     if (slot) {
 #pragma unroll
@@ -811,26 +936,44 @@ __device__ void exchange_serial(const FitDesc& f, int iseq, int it, bool burn, c
   }
 }
 
+// close_k = 0: the exchange of iteration clk.it + it_off of fits without
+// xfused.  close_k = K > 0: the closing launch of a segment of K iterations
+// (clk.xit is its first): for each xfused fit, the exchange of the segment's
+// last iteration from the set that launch wrote, into the first set; or, for
+// a fit whose last iteration came earlier in the segment, a copy of the state
+// that the launch after it left in the second set.  Its first thread then
+// clears the queue heads and moves the clock on.
 __global__ void __launch_bounds__(64) k_exchange(const FitDesc* fits, const WorkItem* xwork, XcdRanges xr,
-                                                 RunClock* clk, TraceBuf tr) {
-  const int it = __atomic_load_n(&clk->xit, __ATOMIC_RELAXED);
+                                                 RunClock* clk, TraceBuf tr, int it_off, int close_k) {
+  const int base = close_k > 0 ? __atomic_load_n(&clk->xit, __ATOMIC_RELAXED)
+                               : __atomic_load_n(&clk->it, __ATOMIC_RELAXED);
   const int xb = (int)(blockIdx.x % NXCD);
   const int wv = xr.off[xb] + (int)(blockIdx.x / NXCD);
   if (wv < xr.off[xb + 1]) {
     const WorkItem w = xwork[wv];
     const FitDesc& f = wave_fit(fits, w.fit);
-    if (it < f.nburn + f.niter) {
-      const bool burn = it < f.nburn;
-      if (f.nbeta <= 64) exchange_wave(f, w.idx, it, burn, tr);
-      else exchange_serial(f, w.idx, it, burn, tr);
+    const int lf = f.nburn + f.niter - 1;   // the fit's last iteration
+    if (close_k == 0) {
+      const int it = base + it_off;
+      if (it <= lf) {
+        if (f.nbeta <= 64) exchange_wave(f, w.idx, it, true, f.pcur, f.chi2c, f.tcur, f.pcur, f.chi2c, f.tcur, tr);
+        else exchange_serial(f, w.idx, it, it < f.nburn, tr);
+      }
+    } else if (lf >= base) {
+      const int last = base + close_k - 1;
+      const int at = lf < last ? lf + 1 - base : close_k - 1;   // launch offset that wrote the state
+      const bool odd = (at & 1) != 0;
+      if (lf >= last || odd)
+        exchange_wave(f, w.idx, last, lf >= last, odd ? f.pcur_b : f.pcur, odd ? f.chi2c_b : f.chi2c,
+                      odd ? f.tcur_b : f.tcur, f.pcur, f.chi2c, f.tcur, tr);
     }
   }
-  // The step launches of this iteration have finished and no work-group of
-  // this launch reads clk->it or the queue heads: one thread sets them up
-  // for the next iteration.
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    for (int q = 0; q < NQSLOT * NXCD; ++q) clk->qhead[q] = 0;
-    __atomic_store_n(&clk->it, it + 1, __ATOMIC_RELAXED);
+  if (close_k > 0 && blockIdx.x == 0 && threadIdx.x == 0) {
+    for (int q = 0; q < NQSLOT * NXCD; ++q) {
+      clk->qhead[0][q] = 0;
+      clk->qhead[1][q] = 0;
+    }
+    __atomic_store_n(&clk->it, base + close_k, __ATOMIC_RELAXED);
   }
 }
 
@@ -931,13 +1074,13 @@ __device__ void iter_quad_wave(const FitDesc& f, int iseq0, int it, const TraceB
 
 template <int NQ>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7)))
-k_iter_quad(const FitDesc* fits, const WorkItem* work, int nwork, RunClock* clk, TraceBuf tr) {
+k_iter_quad(const FitDesc* fits, const WorkItem* work, int nwork, RunClock* clk, TraceBuf tr, int it_off) {
   const int wv = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
   if (wv >= nwork) return;
   pass_clock(clk);
   const WorkItem w = work[wv];
   const FitDesc& f = wave_fit(fits, w.fit);
-  const int it = __builtin_amdgcn_readfirstlane(load_clock(clk));
+  const int it = __builtin_amdgcn_readfirstlane(load_clock(clk, it_off));
   if (it < f.nburn + f.niter) iter_quad_wave<NQ>(f, w.idx, it, tr);
 }
 
@@ -991,7 +1134,7 @@ static void launch_step_form(const StepLaunch& L, const TraceBuf& tr, hipStream_
   constexpr size_t static_lds = sizeof(float2) * WPB * RING + sizeof(float4) * 2 * BESSEL_TAB_N;
   const size_t dyn = occupancy_lds(L, WPB, static_lds);
   const int g = PERSIST ? L.grid : xcd_grid(L.xr, WPB);
-  k_step_eht<M, WPB, PERSIST><<<g, 64 * WPB, dyn, s>>>(L.fits, L.work, L.xr, L.clk, tr, L.qslot);
+  k_step_eht<M, WPB, PERSIST><<<g, 64 * WPB, dyn, s>>>(L.fits, L.work, L.xr, L.clk, tr, L.qslot, L.it_off);
 }
 
 template <class M>
@@ -1035,9 +1178,9 @@ hipError_t launch_step_quad(int n, const StepLaunch& L, hipStream_t s) {
   const TraceBuf tr{L.tr_dec, L.tr_chi2, L.tr_ex, L.tr_it0, L.tr_nit};
   const int g = cdiv(L.nwork, 256);
   switch (n) {
-    case 2: k_step_quad<2><<<g, 256, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr); break;
-    case 4: k_step_quad<4><<<g, 256, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr); break;
-    case 8: k_step_quad<8><<<g, 256, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr); break;
+    case 2: k_step_quad<2><<<g, 256, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr, L.it_off); break;
+    case 4: k_step_quad<4><<<g, 256, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr, L.it_off); break;
+    case 8: k_step_quad<8><<<g, 256, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr, L.it_off); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -1048,19 +1191,20 @@ hipError_t launch_iter_quad(int n, const StepLaunch& L, hipStream_t s) {
   const TraceBuf tr{L.tr_dec, L.tr_chi2, L.tr_ex, L.tr_it0, L.tr_nit};
   const int g = cdiv(L.nwork, 4);   // one work item per wave, 4 waves per work-group
   switch (n) {
-    case 2: k_iter_quad<2><<<g, 256, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr); break;
-    case 4: k_iter_quad<4><<<g, 256, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr); break;
-    case 8: k_iter_quad<8><<<g, 256, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr); break;
+    case 2: k_iter_quad<2><<<g, 256, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr, L.it_off); break;
+    case 4: k_iter_quad<4><<<g, 256, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr, L.it_off); break;
+    case 8: k_iter_quad<8><<<g, 256, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr, L.it_off); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
 }
 
-// With no sequence left to exchange (every fit's exchange ran inside its step
-// launch) one work-group still runs, to hand the clock to the next iteration.
+// A closing launch with no sequence to exchange still runs one work-group, to
+// move the clock on.
 hipError_t launch_exchange(const StepLaunch& L, const WorkItem* xwork, const XcdRanges& xr, hipStream_t s) {
   const TraceBuf tr{L.tr_dec, L.tr_chi2, L.tr_ex, L.tr_it0, L.tr_nit};
-  k_exchange<<<xr.off[NXCD] > 0 ? xcd_grid(xr, 1) : 1, 64, 0, s>>>(L.fits, xwork, xr, L.clk, tr);
+  k_exchange<<<xr.off[NXCD] > 0 ? xcd_grid(xr, 1) : 1, 64, 0, s>>>(L.fits, xwork, xr, L.clk, tr, L.it_off,
+                                                                     L.close_k);
   return hipGetLastError();
 }
 

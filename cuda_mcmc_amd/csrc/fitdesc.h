@@ -80,6 +80,15 @@ struct FitDesc {
   float* pcur; float* chi2c; float* pstp;
   int* n_cnt; int* n_acpt; uint32_t* hbits; int* n_exch; int* sumh;
   int* tcur; int* flag; float* ptent; int* ptentn;
+  // xfused = 1: the replica exchange of iteration it - 1 runs at the start of
+  // the step launch of iteration it (every wave of a sequence computes the
+  // sequence's swaps and takes its own slot's result), so no exchange launch
+  // separates two step launches.  Launch i of a segment then writes the
+  // swapped state (pcur, chi2c, tcur) to the first set of arrays for even i
+  // and to the second (pcur_b, chi2c_b, tcur_b) for odd i, and reads launch
+  // i - 1's; the segment's closing launch leaves it in the first set.
+  int xfused;
+  float* pcur_b; float* chi2c_b; int* tcur_b;
 
   // ---- outputs
   float* pout; int* tout; float* chi2;
@@ -105,23 +114,24 @@ struct XcdRanges {
   int off[NXCD + 1];   // range x = [off[x], off[x + 1])
 };
 
-// Iteration clock shared by the step and exchange kernels so that a cycle of
-// launches can be captured once in a hipGraph and replayed.  Each kind of
-// launch reads its own field and writes the other kind's, which no running
-// work-group reads (launches on one stream never overlap):
-//   step launch:     it = clk.it;  one thread stores clk.xit = it
-//   exchange launch: it = clk.xit; one thread stores clk.it = it + 1 and
-//                    clears the queue heads for the next step launches
-// so no launch has to count its finished work-groups.  (A sequence group
-// whose fits all exchange inside their step launch, k_iter_quad, still ends
-// its iteration with an exchange launch, of one work-group, for the clock:
-// counting the step launch's finished waves in one device-scope atomic
-// instead made those launches slower by more than that launch costs.)
+// Iteration clock of a sequence group.  The host launches the group's
+// iterations in segments of K (one hipGraph replay, or the iterations of one
+// run call): launch i of a segment (i = 0 .. K-1) runs iteration
+// clk.it + i, the offset i being a kernel argument, and the segment ends with
+// one exchange launch that finishes the segment's last exchange and moves the
+// clock on by K.  Each kind of launch reads its own field and writes the
+// other kind's, which no running work-group reads (launches on one stream
+// never overlap):
+//   step launch:     it = clk.it + i;  one thread stores clk.xit = clk.it
+//   closing launch:  it = clk.xit + K - 1;  one thread stores clk.it = clk.xit + K
+// so no launch has to count its finished work-groups.  The persistent step
+// kernels' queue heads alternate between two sets by the parity of i: launch
+// i clears the set that launch i + 1 uses, the closing launch clears both.
 constexpr int NQSLOT = 6;
 struct RunClock {
-  int it;                     // iteration of the next step launches (burn-in first, then sampling)
-  int xit;                    // iteration of the next exchange launch
-  int qhead[NQSLOT * NXCD];   // work-queue heads of the persistent step kernels, per XCD range
+  int it;                        // first iteration of the current segment (burn-in first, then sampling)
+  int xit;                       // copy of it for the closing exchange launch
+  int qhead[2][NQSLOT * NXCD];   // work-queue heads of the persistent step kernels, per XCD range
 };
 
 struct WorkItem {

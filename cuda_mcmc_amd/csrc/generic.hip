@@ -148,13 +148,13 @@ __device__ unsigned user_one_chain(const FitDesc& f, int ch, int it, const Trace
 // One lane per chain; every fit's chain list is padded to whole waves, so a
 // wave's lanes share one fit and its counters take one atomic per wave.
 __global__ void __launch_bounds__(64) k_step_user(const FitDesc* fits, const WorkItem* work, int nwork,
-                                                  RunClock* clk, TraceBuf tr) {
+                                                  RunClock* clk, TraceBuf tr, int it_off) {
   const int g = blockIdx.x * blockDim.x + threadIdx.x;
   if (__builtin_amdgcn_readfirstlane(g) >= nwork) return;
   const WorkItem w = work[g];
   const FitDesc& f = wave_fit(fits, w.fit);
   pass_clock(clk);
-  const int it = __builtin_amdgcn_readfirstlane(load_clock(clk));
+  const int it = __builtin_amdgcn_readfirstlane(load_clock(clk, it_off));
   if (it >= f.nburn + f.niter) return;
   const bool live = w.idx >= 0;
   const unsigned nev = live ? user_one_chain(f, w.idx, it, tr) : 0u;
@@ -173,7 +173,7 @@ hipError_t launch_eval_user(const FitDesc* fits, const WorkItem* work, int nwork
 hipError_t launch_step_user(const StepLaunch& L, hipStream_t s) {
   if (L.nwork <= 0) return hipSuccess;
   const TraceBuf tr{L.tr_dec, L.tr_chi2, L.tr_ex, L.tr_it0, L.tr_nit};
-  k_step_user<<<cdiv(L.nwork, 64), 64, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr);
+  k_step_user<<<cdiv(L.nwork, 64), 64, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr, L.it_off);
   return hipGetLastError();
 }
 

@@ -141,11 +141,13 @@ class Session:
     force_generic (run imodel 1/2 through the compiled-in model.cuh),
     waves_per_simd (cap on the native step kernel's resident waves per SIMD),
     streams (iterate groups of sequences on this many overlapping streams;
-    0 = two in the grid form, one in the persistent form).
+    0 = two in the grid form, one in the persistent form),
+    exchange_launch (1: the ring+Gaussian fits exchange in a launch of their
+    own after every step launch instead of inside the next one).
     """
 
     def __init__(self, fits, lib=None, waves_per_block=0, persistent=-1, graph_iters=0, trace_iters=0,
-                 force_generic=False, waves_per_simd=0, streams=0):
+                 force_generic=False, waves_per_simd=0, streams=0, exchange_launch=0):
         self.L = _lib.load(lib)
         self.fits = list(fits)
         self._specs = (MhreFitSpec * len(self.fits))(*[f.spec() for f in self.fits])
@@ -156,6 +158,7 @@ class Session:
         self.opt.force_generic = int(bool(force_generic))
         self.opt.waves_per_simd = int(waves_per_simd)
         self.opt.streams = int(streams)
+        self.opt.exchange_launch = int(bool(exchange_launch))
         self.trace = None      # single fit: dict of arrays; several fits: one dict per fit
         if trace_iters:
             T = int(trace_iters)

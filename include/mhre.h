@@ -143,6 +143,12 @@ typedef struct MhreOptions {
                                    overlap; 0 = default (2 in the grid form, 1
                                    in the persistent form).  Results do not
                                    depend on it. */
+  int exchange_launch;          /* 1: the ring+Gaussian fits' replica exchange
+                                   runs in a launch of its own after every step
+                                   launch; 0 = default: inside the next step
+                                   launch (ladders of at most 64 rungs), which
+                                   saves a launch per iteration.  Results do
+                                   not depend on it. */
 } MhreOptions;
 
 typedef struct MhreStats {
@@ -161,9 +167,10 @@ int mhre_session_create(const MhreFitSpec *fits, int nfits, const MhreOptions *o
 int mhre_session_run(void *session, int niter);
 /* Run niter iterations timing every launch with HIP events on the stream of
  * its sequence group; returns the average device time of one group's step
- * launch(es) and of its exchange launch per iteration, in milliseconds (the
- * launches of different groups overlap, so these do not add up to the
- * iteration time). */
+ * launch(es) per iteration and of its exchange launches per iteration (those
+ * that run every iteration, plus each segment's closing launch spread over
+ * the segment's iterations), in milliseconds (the launches of different
+ * groups overlap, so these do not add up to the iteration time). */
 int mhre_session_time_kernels(void *session, int niter, double *step_ms, double *exch_ms);
 /* Choose the step kernels' launch form (waves per work-group 1/2/4, grid or
  * persistent work queue) by timing each candidate on `iters` real

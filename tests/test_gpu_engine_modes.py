@@ -10,7 +10,11 @@ change a single bit:
 * the no-recording mode (pout/tout/chi2 NULL) leaves the final state unchanged;
 * sequence groups iterated on separate streams vs one stream;
 * the autotuner, which times every launch form on real iterations, leaves
-  the chains as a plain run does.
+  the chains as a plain run does;
+* the replica exchange run at the start of the next step launch (the
+  default for ring+Gaussian ladders of at most 64 rungs) vs an exchange launch
+  after every step launch, over runs cut into segments of every parity and
+  fits that end inside a segment.
 
 The generic model.cuh path runs the same EHT fit as the native kernel: its
 chi^2 agrees to 2e-6 relative.
@@ -156,3 +160,38 @@ def test_sequence_groups_on_streams_bit_identical(streams):
         got = _run(make(), streams=streams, **opt)
         for a, b in zip(alone, got):
             _same(a, b)
+
+
+def _run_segments(fits, segments, **opt):
+    with Session(fits, trace_iters=sum(segments), **opt) as s:
+        for n in segments:
+            s.run(n)
+        s.download()
+        trace = s.trace
+    return fits, trace
+
+
+@pytest.mark.parametrize("opt", [dict(), dict(persistent=1), dict(graph_iters=3), dict(streams=1),
+                                 dict(streams=3, waves_per_block=2)])
+def test_exchange_in_step_launch_matches_exchange_launch(opt):
+    # segments of 1, 2, 3, 5 and 9 iterations (20 in all).  The fits end at
+    # iterations 19 (the last segment's closing launch swaps), 8 (its exchange
+    # runs at offset 3 of the segment 6..10, so the closing launch copies the
+    # second state set back), 7 (exchange at offset 2: already in the first
+    # set), 11 (offset 0 of the last segment, exchange at offset 1), and a
+    # 13-parameter fit at 8; a 70-rung ladder keeps its exchange launch of
+    # every iteration in the same session.
+    def make():
+        return [_fit(700, 200, 111, nb=8, ns=16, nburn=7, niter=13),
+                _fit(300, 90, 112, nb=5, ns=12, nburn=4, niter=5),
+                _fit(500, 0, 113, nb=1, ns=20, nburn=4, niter=4),
+                _fit(400, 120, 114, nb=40, ns=3, nburn=6, niter=6),
+                _fit(200, 60, 115, nb=70, ns=2, nburn=5, niter=5),
+                eht_fit(synthdata.ring_gauss_dataset(nvis=300, ncph=90, seed=116), 6, 10, nburn=5, niter=4,
+                        seed=117, model="sgra13")]
+    ref, tr_ref = _run_segments(make(), [20], exchange_launch=1)
+    got, tr_got = _run_segments(make(), [1, 2, 3, 5, 9], **opt)
+    for a, b, ta, tb in zip(ref, got, tr_ref, tr_got):
+        _same(a, b)
+        for k in ("dec", "chi2", "ex"):
+            np.testing.assert_array_equal(ta[k], tb[k], err_msg=k)
