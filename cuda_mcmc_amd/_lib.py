@@ -147,6 +147,8 @@ def _declare(L):
     L.mhre_session_download.argtypes = [vp, C.c_int]
     L.mhre_session_stats.restype = C.c_int
     L.mhre_session_stats.argtypes = [vp, C.POINTER(MhreStats)]
+    L.mhre_session_fit_info.restype = C.c_int
+    L.mhre_session_fit_info.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.c_int]
     L.mhre_session_destroy.restype = C.c_int
     L.mhre_session_destroy.argtypes = [vp]
     L.mhre_run_batch.restype = C.c_int

@@ -25,6 +25,7 @@
 #define MHRE_NO_MODEL_MACROS 1
 #include "ccalcmodel.h"
 #include "fitdesc.h"
+#include "eht_model.h"
 #include "engine.h"
 #include "grad.h"
 #include "mhre.h"
@@ -114,6 +115,7 @@ struct FitHost {
   size_t o_pout, o_tout, o_chi2, o_datm, o_chi2m, o_mc;
   size_t o_pcur_b = 0, o_chi2c_b = 0, o_tcur_b = 0;   // second state set (xfused)
   bool xfused = false;           // exchange inside the next step launch (FitDesc::xfused)
+  bool hwphase = false;          // FitDesc::hwphase
   bool has_datm = false;
   bool record = true;            // pout/tout/chi2 given
   FitDesc d{};
@@ -158,6 +160,48 @@ int validate(const MhreFitSpec& s, int idx) {
 // Native EHT inputs from the reference arrays (src/imgpu.py:759-831 layout):
 // coor = [u0,v0,...] (+ [u1,v1,u2,v2] per closure when cpExt = 1),
 // dat/std2r = [amplitudes, closures], icoor = closure triplets, idat = [nvis, nchi2m, cpExt].
+// Largest |shift phase|, in revolutions, that the ring+Gaussian kernels can
+// meet for this fit: the parameters' magnitudes over the prior box and the
+// uploaded chain values (fixed parameters keep theirs; varied ones only take
+// values inside the box), times the longest baseline of the packed
+// visibilities.  Sgra9 (eht_model.h): ring shift d = ecc (1 - rq) Re, Gaussian
+// shift (rq - ecc (1 - rq)) Re; Sgra13: d also times cos(eta - th), Gaussian
+// shift gr Re.  A non-finite input gives a non-finite bound, and so does a
+// spec without a prior (the gradient's data handle, which samples nothing).
+double eht_phase_bound(const FitHost& h) {
+  const MhreFitSpec& s = h.s;
+  if (!s.pmint || !s.pmaxt) return INFINITY;
+  const size_t N = (size_t)s.nptot, C = (size_t)s.nbeta * s.nseq;
+  auto range = [&](int k, double* lo, double* hi) {
+    *lo = std::min((double)s.pmint[k], (double)s.pmaxt[k]);
+    *hi = std::max((double)s.pmint[k], (double)s.pmaxt[k]);
+    if (!std::isfinite(*lo) || !std::isfinite(*hi)) *hi = INFINITY;
+    if (s.pcur)
+      for (size_t c = 0; c < C; ++c) {
+        const double v = s.pcur[c * N + k];
+        if (!std::isfinite(v)) *hi = INFINITY;
+        *lo = std::min(*lo, v);
+        *hi = std::max(*hi, v);
+      }
+  };
+  auto amax = [&](int k) {
+    double lo, hi;
+    range(k, &lo, &hi);
+    return std::max(std::fabs(lo), std::fabs(hi));
+  };
+  double rlo, rhi;
+  range(2, &rlo, &rhi);
+  const double one_minus_rq = std::max(std::fabs(1.0 - rlo), std::fabs(1.0 - rhi));
+  const double re = amax(1) * UAS2NRAD;
+  const double e = amax(3) * one_minus_rq;   // |ecc (1 - rq)|
+  const double ring = e * re;
+  const double gauss = (h.kind == K_SGRA9 ? amax(2) + e : amax(5)) * re;
+  double rho = 0.0;
+  for (int i = 0; i < h.nvis_k; ++i)
+    rho = std::max(rho, std::hypot((double)h.vis[4 * i], (double)h.vis[4 * i + 1]));
+  return std::max(ring, gauss) * rho;
+}
+
 int pack_eht(FitHost& h, int idx) {
   const MhreFitSpec& s = h.s;
   const int need = h.kind == K_SGRA9 ? 9 : 13;
@@ -267,6 +311,9 @@ int pack_eht(FitHost& h, int idx) {
     mx = std::nextafter(mx * (1.0f + 1e-6f), INFINITY);
     memcpy(&h.tinfo[2 * t + 1], &mx, 4);
   }
+  // The shift phases need no reduction before the hardware sine when they
+  // are bounded well inside its input domain (|h| <= 256 revolutions).
+  h.hwphase = eht_phase_bound(h) <= 64.0 && !env_flag("MHRE_REDUCE_PHASES");
   return MHRE_OK;
 }
 
@@ -990,6 +1037,7 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
     d.n_exch = at<int>(S, h.o_nexch); d.sumh = at<int>(S, h.o_sumh); d.tcur = at<int>(S, h.o_tcur);
     d.flag = at<int>(S, h.o_flag); d.ptent = at<float>(S, h.o_ptent); d.ptentn = at<int>(S, h.o_ptentn);
     d.xfused = h.xfused ? 1 : 0;
+    d.hwphase = h.hwphase ? 1 : 0;
     if (h.xfused) {
       d.pcur_b = at<float>(S, h.o_pcur_b); d.chi2c_b = at<float>(S, h.o_chi2c_b); d.tcur_b = at<int>(S, h.o_tcur_b);
     }
@@ -1737,6 +1785,18 @@ int mhre_session_download(void* session, int model_data) {
 int mhre_session_stats(void* session, MhreStats* st) {
   if (!session) FAIL(MHRE_EINVAL, "NULL session");
   return stats(static_cast<Session*>(session), st);
+}
+
+int mhre_session_fit_info(const void* session, int fit, int* info, int ninfo) {
+  if (!session || !info || ninfo < 4) FAIL(MHRE_EINVAL, "mhre_session_fit_info: NULL argument or ninfo < 4");
+  const Session* S = static_cast<const Session*>(session);
+  if (fit < 0 || fit >= (int)S->fits.size()) FAIL(MHRE_EINVAL, "mhre_session_fit_info: no fit %d", fit);
+  const FitHost& h = S->fits[fit];
+  info[0] = h.kind;
+  info[1] = (h.xfused || quad_fused(h)) ? 1 : 0;
+  info[2] = h.hwphase ? 1 : 0;
+  info[3] = (int)S->groups.size();
+  return MHRE_OK;
 }
 
 int mhre_session_destroy(void* session) {

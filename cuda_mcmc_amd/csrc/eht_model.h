@@ -69,8 +69,9 @@ __device__ inline void ring_coefs(double Rex, double rq, double fade, double e, 
 // any_large == false: the caller guarantees ye <= 64 on every lane, so only
 // the y <= 64 evaluation bc of B, C runs (BesselPoly or BesselTab).
 // bc takes its argument in units of 1/BC::SCALE (the scale is folded into se
-// once per chain, outside the visibility loop).
-template <class BC>
+// once per chain, outside the visibility loop).  HW: the shift phase is known
+// to lie inside the hardware sine's input domain (sincos_phase).
+template <bool HW, class BC>
 __device__ inline float2 ring_vis(const RingInv& q, float U, float rho2, bool any_large, const BC& bc) {
   constexpr float S = BC::SCALE;
   const float ye = (q.se * S) * rho2;
@@ -90,7 +91,7 @@ __device__ inline float2 ring_vis(const RingInv& q, float U, float rho2, bool an
   const float rei = q.mBi * Bi;
   const float imi = q.mCi * U * Ci;
   float sd, cd;
-  sincosrev(q.kdr * U, &sd, &cd);
+  sincos_phase<HW>(q.kdr * U, &sd, &cd);
   float2 F;
   F.x = fmaf(imi, sd, fmaf(-rei, cd, ree));
   F.y = fmaf(-imi, cd, fmaf(-rei, sd, ime));
@@ -160,13 +161,14 @@ struct Sgra9 {
   // Visibility at (u, v) with rho2 = u^2 + v^2.  The Gaussian exponent
   // gaU U^2 + gaV V^2 is formed as gaV rho2 + (gaU - gaV) U^2.  Without a
   // Gaussian (cg = 0) its term is added as zero, so the code has no branch.
-  template <class BC = BesselPoly>
+  // HW: both shift phases lie inside the hardware sine's input domain.
+  template <class BC = BesselPoly, bool HW = false>
   __device__ static float2 vis(const Inv& q, float u, float v, float rho2, bool any_large, const BC& bc = BC()) {
     const float U = fmaf(u, q.cth, v * q.sth);
-    float2 out = ring_vis(q.r, U, rho2, any_large, bc);
+    float2 out = ring_vis<HW>(q.r, U, rho2, any_large, bc);
     const float ga = __builtin_amdgcn_exp2f(-fmaf(q.gaD, U * U, q.gaV * rho2));
     float sg, cgs;
-    sincosrev(q.kgr * U, &sg, &cgs);
+    sincos_phase<HW>(q.kgr * U, &sg, &cgs);
     const float gw = q.r.cg * ga;
     out.x = fmaf(gw, cgs, out.x);
     out.y = fmaf(gw, sg, out.y);
@@ -205,14 +207,14 @@ struct Sgra13 {
 
   // Visibility at (u, v) with rho2 = u^2 + v^2 (see Sgra9::vis).  The ring
   // is skipped for a Gaussian-only source, as in the reference.
-  template <class BC = BesselPoly>
+  template <class BC = BesselPoly, bool HW = false>
   __device__ static float2 vis(const Inv& q, float u, float v, float rho2, bool any_large, const BC& bc = BC()) {
     float2 out = make_float2(0.0f, 0.0f);
-    if (!q.gonly) out = ring_vis(q.r, fmaf(u, q.cth, v * q.sth), rho2, any_large, bc);
+    if (!q.gonly) out = ring_vis<HW>(q.r, fmaf(u, q.cth, v * q.sth), rho2, any_large, bc);
     const float Ub = fmaf(u, q.cb, v * q.sb);
     const float ga = __builtin_amdgcn_exp2f(-fmaf(q.gaD, Ub * Ub, q.gaV * rho2));
     float sg, cgs;
-    sincosrev(fmaf(q.kgu, u, q.kgv * v), &sg, &cgs);
+    sincos_phase<HW>(fmaf(q.kgu, u, q.kgv * v), &sg, &cgs);
     const float gw = q.r.cg * ga;
     out.x = fmaf(gw, cgs, out.x);
     out.y = fmaf(gw, sg, out.y);

@@ -139,8 +139,10 @@ __device__ inline float2 ring_at(const float2* ring, int off) {
 // The fp32 terms are summed per lane in fp32 over blocks of 8 tiles (at most
 // ~10 terms), each block sum is added to a per-lane fp64 total, and the
 // totals are summed across the wave.
-// OUT: also write datm/chi2m of chain ch in the reference layout.
-template <class M, bool OUT>
+// OUT: also write datm/chi2m of chain ch in the reference layout.  HW: the
+// fit's shift phases are known to stay inside the hardware sine's input
+// domain (FitDesc::hwphase), so they are not reduced first.
+template <class M, bool OUT, bool HW = false>
 __device__ double eht_chi2_wave(const EhtView& v, const typename M::Inv& q, float2* ring, const BesselTab& bt,
                                 int ch) {
   const int lane = lane_id();
@@ -177,7 +179,7 @@ __device__ double eht_chi2_wave(const EhtView& v, const typename M::Inv& q, floa
     int4 ix = buf_load_i4(cb, l16, (uint32_t)c0 * 16u);
     float4 ow = buf_load_f4(wb, l16, (uint32_t)c0 * 16u);
     const bool any_large = __float_as_uint(r2cur) > r2_thr;
-    const float2 V = M::vis(q, d.x, d.y, r2, any_large, bt);
+    const float2 V = M::template vis<BesselTab, HW>(q, d.x, d.y, r2, any_large, bt);
     const float amp = __builtin_amdgcn_sqrtf(fmaf(V.x, V.x, V.y * V.y));
     const float r = d.z - amp;
     part = fmaf(r * d.w, r, part);   // zero weight past the amplitude data
@@ -487,7 +489,9 @@ __device__ void step_one_chain(const FitDesc& f, int ch, int it, int it_off, flo
       gather_params<M>(pl, p);
       const typename M::Trig tn = M::trig_uses(t) ? M::trig(p) : tg;
       const typename M::Inv q = M::prepare(p, tn);
-      c2 = eht_chi2_wave<M, false>(view, q, ring, bt, ch);
+      // one copy of the chi^2 loop per phase form (the branch is per wave)
+      c2 = f.hwphase ? eht_chi2_wave<M, false, true>(view, q, ring, bt, ch)
+                     : eht_chi2_wave<M, false, false>(view, q, ring, bt, ch);
       ++nev;
       const double x = -0.5 * beta * (c2 - (double)chi2c);
       if (bcast(lu_l, s) <= x) {

@@ -50,6 +50,9 @@ struct FitDesc {
   int hwords;            // words per acceptance history: ceil(nadj / 32)
   int xlanes;            // exchange: lanes per sequence (a power of two >= nbeta, or 64),
                          // so one wave swaps 64 / xlanes sequences side by side
+  int hwphase;           // ring+Gaussian step kernel: every shift phase (revolutions) is
+                         // bounded by 64 over the prior box, the chains' values and the
+                         // data's baselines, so the hardware sine needs no reduction
   long long ndatm;       // length of datm (head nchain*ndat, then the phase tail)
   long long tr_dec;      // decision trace: this fit's first entry in the session's
   long long tr_ex;       // dec/chi2 and exchange trace arrays (fits in session order)

@@ -114,6 +114,25 @@ __device__ inline void sincosrev(float h, float* s, float* c) {
   *c = __builtin_amdgcn_cosf(r);
 }
 
+// The same for |h| <= 255, inside the hardware's input domain, without the
+// reduction: the instructions reduce their argument themselves, exactly
+// (measured on MI355X against float64 over |h| <= 0.5, 1, 2, 4, 16, 64 and 255
+// revolutions: max abs error 1.25e-7 with or without the explicit step).
+// Two instructions fewer per evaluation; the host enables it only for fits
+// whose phases it has bounded below that (FitDesc::hwphase).
+__device__ inline void sincosrev_hw(float h, float* s, float* c) {
+  *s = __builtin_amdgcn_sinf(h);
+  *c = __builtin_amdgcn_cosf(h);
+}
+
+// sincosrev, or without the reduction (HW) for arguments known to lie inside
+// the hardware's input domain.
+template <bool HW>
+__device__ inline void sincos_phase(float h, float* s, float* c) {
+  if (HW) sincosrev_hw(h, s, c);
+  else sincosrev(h, s, c);
+}
+
 // atan2(y, x) in (-pi, pi]: t = min/max in [0,1], atan(t) = t P(t^2), octant fix-up.
 __device__ inline float atan2_fast(float y, float x) {
   const float ax = fabsf(x), ay = fabsf(y);

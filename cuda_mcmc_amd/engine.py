@@ -215,6 +215,16 @@ class Session:
         _lib.check(self.L, self.L.mhre_session_stats(self.h, C.byref(st)), "mhre_session_stats")
         return {k: getattr(st, k) for k, _ in MhreStats._fields_}
 
+    def fit_info(self, i=0):
+        """How fit i is run: its step kernel ("sgra9", "sgra13", "user",
+        "quad"), whether its replica exchange runs inside a step launch,
+        whether its shift phases skip the reduction before the hardware sine,
+        and the session's sequence groups."""
+        info = (C.c_int * 4)()
+        _lib.check(self.L, self.L.mhre_session_fit_info(self.h, int(i), info, 4), "mhre_session_fit_info")
+        return dict(kernel=("sgra9", "sgra13", "user", "quad")[info[0]], exchange_in_step=bool(info[1]),
+                    hw_phase=bool(info[2]), groups=int(info[3]))
+
     def close(self):
         if self.h:
             self.L.mhre_session_destroy(self.h)

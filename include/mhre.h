@@ -185,6 +185,14 @@ int mhre_session_sync(void *session);
  * model_data != 0 also fill datm/chi2m for the final state. */
 int mhre_session_download(void *session, int model_data);
 int mhre_session_stats(void *session, MhreStats *stats);
+/* How fit `fit` of the session is run (no reference counterpart):
+ * info[0] = step kernel (0 native 9-parameter ring+Gaussian, 1 native
+ * 13-parameter, 2 the compiled-in model.cuh, 3 quadratic systems),
+ * info[1] = 1 if its replica exchange runs inside a step launch (no exchange
+ * launch per iteration), info[2] = 1 if its ring+Gaussian shift phases skip
+ * the reduction before the hardware sine (bounded by 64 revolutions),
+ * info[3] = the session's sequence groups.  ninfo >= 4. */
+int mhre_session_fit_info(const void *session, int fit, int *info, int ninfo);
 int mhre_session_destroy(void *session);
 /* create + run(nburn + niter) + download + destroy. */
 int mhre_run_batch(const MhreFitSpec *fits, int nfits, const MhreOptions *opt, MhreStats *stats);
