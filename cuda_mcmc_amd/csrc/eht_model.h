@@ -29,6 +29,11 @@ struct RingInv {
   float cr, cg;            // ring and Gaussian mixing weights (Zsp folded in)
   int gauss;               // Gaussian present
   float mBe, mCe, mBi, mCi;   // cr * (kBe, kCe, kBi, kCi): the ring term already mixed
+  // float bits of 64 / (se max(1, rq2)): a tile whose largest u^2 + v^2 is
+  // above it needs the large-argument Bessel branch (the larger of the two
+  // arguments, max(1, rq^2) se rho^2, exceeds 64; rq > 1 makes the inner one
+  // the larger)
+  uint32_t r2thr;
 };
 
 // The mixed ring coefficients, once cr and the k's are known (rounded to
@@ -61,6 +66,7 @@ __device__ inline void ring_coefs(double Rex, double rq, double fade, double e, 
   q->kCi = (float)(-2.0 * D_PI * D_PI * (h - h0) * rq * Rin * Rin * Rin);
   q->kd = (float)(2.0 * D_PI * d);
   q->kdr = (float)d;
+  q->r2thr = __float_as_uint(64.0f / (q->se * fmaxf(1.0f, q->rq2)));
 }
 
 // Ring visibility cr (Fex - Fin) at rotated coordinate U and squared
@@ -147,15 +153,31 @@ struct Sgra9 {
     mix_ring(&q.r);
   }
 
-  __device__ static Inv prepare(const float* p, const Trig& tg) {
+  // The ring group (Re, rq, ecc, fade): the costly part of prepare (float64,
+  // one division).  The step kernel keeps the current state's copy and
+  // recomputes it only for proposals that move one of these parameters.
+  struct RingPart {
+    RingInv r;
+    float kg, kgr;
+  };
+  __device__ static bool ring_uses(int t) { return t >= 1 && t <= 4; }
+  __device__ static RingPart ring_part(const float* p, const Trig&) {
     Inv q;
     ring(q, p);
+    return RingPart{q.r, q.kg, q.kgr};
+  }
+  __device__ static Inv assemble(const float* p, const Trig& tg, const RingPart& rp) {
+    Inv q;
+    q.r = rp.r;
+    q.kg = rp.kg;
+    q.kgr = rp.kgr;
     q.cth = tg.cth;
     q.sth = tg.sth;
     gaussian(q, p);
     mixing(q, p);
     return q;
   }
+  __device__ static Inv prepare(const float* p, const Trig& tg) { return assemble(p, tg, ring_part(p, tg)); }
   __device__ static Inv prepare(const float* p) { return prepare(p, trig(p)); }
 
   // Visibility at (u, v) with rho2 = u^2 + v^2.  The Gaussian exponent
@@ -202,7 +224,15 @@ struct Sgra13 {
   };
   __device__ static bool trig_uses(int t) { return t >= 9; }
   __device__ static Trig trig(const float* p);
-  __device__ static Inv prepare(const float* p, const Trig& tg);
+  // The ring group (Re, rq, ecc, fade and, through cos(eta - th), eta and
+  // th): see Sgra9::RingPart.
+  struct RingPart {
+    RingInv r;
+  };
+  __device__ static bool ring_uses(int t) { return (t >= 1 && t <= 4) || t == 11 || t == 12; }
+  __device__ static RingPart ring_part(const float* p, const Trig& tg);
+  __device__ static Inv assemble(const float* p, const Trig& tg, const RingPart& rp);
+  __device__ static Inv prepare(const float* p, const Trig& tg) { return assemble(p, tg, ring_part(p, tg)); }
   __device__ static Inv prepare(const float* p) { return prepare(p, trig(p)); }
 
   // Visibility at (u, v) with rho2 = u^2 + v^2 (see Sgra9::vis).  The ring
@@ -240,12 +270,19 @@ __device__ inline Sgra13::Trig Sgra13::trig(const float* p) {
   return tg;
 }
 
-__device__ inline Sgra13::Inv Sgra13::prepare(const float* p, const Trig& tg) {
+__device__ inline Sgra13::RingPart Sgra13::ring_part(const float* p, const Trig& tg) {
+  RingPart rp;
+  const double Re = p[1], rq = p[2], ecc = p[3], fade = p[4];
+  ring_coefs(Re * UAS2NRAD, rq, fade, ecc * (1.0 - rq) * tg.ceth, &rp.r);   // d / Re, d = ecc (Re - Ri) cos(eta - th)
+  return rp;
+}
+
+__device__ inline Sgra13::Inv Sgra13::assemble(const float* p, const Trig& tg, const RingPart& rp) {
   Inv q;
-  const double Zsp = p[0], Re = p[1], rq = p[2], ecc = p[3], fade = p[4];
+  const double Zsp = p[0], Re = p[1];
   const double gr = p[5], gax = p[6], aq = p[7], gq = p[8];
   const double Rex = Re * UAS2NRAD;
-  ring_coefs(Rex, rq, fade, ecc * (1.0 - rq) * tg.ceth, &q.r);   // d / Re, d = ecc (Re - Ri) cos(eta - th)
+  q.r = rp.r;
   q.cth = tg.cth;
   q.sth = tg.sth;
   q.cb = tg.cb;

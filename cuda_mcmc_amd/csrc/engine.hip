@@ -165,12 +165,10 @@ __device__ double eht_chi2_wave(const EhtView& v, const typename M::Inv& q, floa
   int kleft = v.cgroup;
   int2 ti = ldg(v.tinfo, 1);
   float r2cur = __int_as_float(ldg(v.tinfo, 0).y);
-  // tile t needs the large-argument branch when the larger of the two Bessel
-  // arguments, max(1, rq^2) * se * r2, exceeds 64 (rq > 1 makes the inner
-  // argument the larger one), i.e. when r2 exceeds 64 / (se max(1, rq^2)):
-  // a compare of the (non-negative) float bits, in scalar registers
-  const uint32_t r2_thr =
-      __builtin_amdgcn_readfirstlane(__float_as_uint(64.0f / (q.r.se * fmaxf(1.0f, q.r.rq2))));
+  // tile t needs the large-argument branch when its largest u^2 + v^2 is
+  // above q.r.r2thr (RingInv): a compare of the (non-negative) float bits, in
+  // scalar registers
+  const uint32_t r2_thr = __builtin_amdgcn_readfirstlane(q.r.r2thr);
   // tile t with its data (d, r2); fetches tile t + 1 into (dn, r2n)
   auto tile = [&](const int t, const float4& d, const float r2, float4& dn, float& r2n) {
     dn = buf_load_f4(vb, l16, (uint32_t)(t + 1) * 1024u);
@@ -320,7 +318,10 @@ __global__ void __launch_bounds__(256) k_eval_eht(const FitDesc* fits, const Wor
 // computes it and keeps what its own slot sl needs: the slot whose row ends
 // up in sl, and sl's chi^2 and label after the swaps.  It also does the
 // exchange launch's per-slot writes for sl (n_exch) and, in slot 0's wave,
-// the sequence's trace entries.
+// the sequence's trace entries.  Every operand of the attempts is
+// wave-uniform (lane reads; the n_exch count and the accept bits stay in
+// scalar registers); labels move with the rows, so sl's label is the one its
+// row started with.
 struct SlotAfterSwaps {
   int src;
   float c2;
@@ -333,9 +334,9 @@ __device__ inline SlotAfterSwaps seq_exchange(const FitDesc& f, int iseq, int sl
   const bool burn = itp < f.nburn;
   const bool slot = lane < B;
   float c2 = slot ? c2s[(size_t)lane * S + iseq] : 0.0f;
-  int tc = slot ? tcs[(size_t)lane * S + iseq] : 0;
+  const int tc0 = slot ? tcs[(size_t)lane * S + iseq] : 0;
   int perm = lane;
-  int nex = 0;
+  int nx = 0;
   if (B > 1) {
     // attempt a's pair pick and log u on lane a, all drawn up front
     int k_l = 0;
@@ -346,25 +347,26 @@ __device__ inline SlotAfterSwaps seq_exchange(const FitDesc& f, int iseq, int sl
       lu_l = log((double)u01(r.y));
     }
     if (lane < B - 1) hb_l = -0.5 * ((double)f.beta[lane + 1] - (double)f.beta[lane]);
-    const bool trace = tr.ex && sl == 0 && itp >= tr.it0 && itp < tr.it0 + tr.nit;
-    for (int a = 0; a < B; ++a) {   // every value below is uniform: lane reads, no shuffles
+    uint64_t accm = 0;   // bit a: attempt a swapped
+    for (int a = 0; a < B; ++a) {
       const int k = bcast(k_l, a);
       const float ck = bcast(c2, k), ck1 = bcast(c2, k + 1);
       const double x = bcast(hb_l, k) * ((double)ck - (double)ck1);
-      const bool acc = bcast(lu_l, a) <= x;
-      if (acc) {
-        const int tk = bcast(tc, k), tk1 = bcast(tc, k + 1);
+      if (bcast(lu_l, a) <= x) {
+        accm |= 1ull << a;
         const int pk = bcast(perm, k), pk1 = bcast(perm, k + 1);
-        if (lane == k) { c2 = ck1; tc = tk1; perm = pk1; if (!burn) ++nex; }
-        if (lane == k + 1) { c2 = ck; tc = tk; perm = pk; }
+        if (lane == k) { c2 = ck1; perm = pk1; }
+        if (lane == k + 1) { c2 = ck; perm = pk; }
+        if (k == sl) ++nx;
       }
-      if (trace && lane == 0)
-        tr.ex[(size_t)f.tr_ex + ((size_t)(itp - tr.it0) * B + a) * S + iseq] = k | ((acc ? 1 : 0) << 16);
     }
+    if (tr.ex && sl == 0 && slot && itp >= tr.it0 && itp < tr.it0 + tr.nit)
+      tr.ex[(size_t)f.tr_ex + ((size_t)(itp - tr.it0) * B + lane) * S + iseq] =
+          k_l | ((int)((accm >> lane) & 1ull) << 16);
   }
-  const int nx = bcast(nex, sl);
   if (!burn && nx && lane == 0) f.n_exch[(size_t)sl * S + iseq] += nx;
-  return SlotAfterSwaps{bcast(perm, sl), bcast(c2, sl), bcast(tc, sl)};
+  const int src = bcast(perm, sl);
+  return SlotAfterSwaps{src, bcast(c2, sl), bcast(tc0, src)};
 }
 
 // One MH iteration of chain ch (all nprm single-parameter proposals) by one
@@ -373,13 +375,25 @@ __device__ inline SlotAfterSwaps seq_exchange(const FitDesc& f, int iseq, int sl
 // the two sets by the parity of it_off); for it past the fit's last iteration
 // only that exchange runs.
 template <class M>
-__device__ void step_one_chain(const FitDesc& f, int ch, int it, int it_off, float2* ring, const BesselTab& bt,
-                               const TraceBuf& tr) {
+__device__ void step_one_chain(const FitDesc& f, int ch, int it, int it_off, float2* ring,
+                               typename M::RingPart* rcache, const BesselTab& bt, const TraceBuf& tr) {
   const int lane = lane_id();
   const int C = f.nchain, P = f.nprm, N = f.nptot;
   const int ibeta = ch / f.nseq;
   const bool burn = it < f.nburn;
   const int nit = f.nburn + f.niter;
+
+  // lane-distributed constants and per-slot state (not moved by the
+  // exchange: read first, so that they arrive while the exchange runs)
+  const int pdl = lane < N ? f.pdescr[lane] : 0;
+  const float pminl = lane < N ? f.pmint[lane] : 0.0f;
+  const float pmaxl = lane < N ? f.pmaxt[lane] : 0.0f;
+  const int ivl = lane < P ? f.ivar[lane] : 0;
+  float stp = lane < P ? f.pstp[(size_t)lane * C + ch] : 0.0f;
+  int cnt = lane < P ? f.n_cnt[(size_t)lane * C + ch] : 0;
+  int nac = lane < P ? f.n_acpt[(size_t)lane * C + ch] : 0;
+  int sh = (burn && lane < P) ? f.sumh[(size_t)lane * C + ch] : 0;
+  const double beta = (double)f.beta[ibeta];
 
   // the chain's state before the step (after the previous exchange), and
   // where the state after the step goes
@@ -431,15 +445,6 @@ __device__ void step_one_chain(const FitDesc& f, int ch, int it, int it_off, flo
     chi2c = f.chi2c[ch];
   }
 
-  // lane-distributed chain state and constants
-  const int pdl = lane < N ? f.pdescr[lane] : 0;
-  const float pminl = lane < N ? f.pmint[lane] : 0.0f;
-  const float pmaxl = lane < N ? f.pmaxt[lane] : 0.0f;
-  const int ivl = lane < P ? f.ivar[lane] : 0;
-  float stp = lane < P ? f.pstp[(size_t)lane * C + ch] : 0.0f;
-  int cnt = lane < P ? f.n_cnt[(size_t)lane * C + ch] : 0;
-  int nac = lane < P ? f.n_acpt[(size_t)lane * C + ch] : 0;
-  int sh = (burn && lane < P) ? f.sumh[(size_t)lane * C + ch] : 0;
   // lane j keeps the history word holding entry cnt % nadj of parameter j
   int hw_idx = 0;
   uint32_t hword = 0u;
@@ -447,17 +452,23 @@ __device__ void step_one_chain(const FitDesc& f, int ch, int it, int it_off, flo
     hw_idx = (cnt % f.nadj) >> 5;
     hword = f.hbits[((size_t)hw_idx * P + lane) * C + ch];
   }
-  const double beta = (double)f.beta[ibeta];
   int flag = 0, tlast = 0;
   float told = 0.0f;
   unsigned nev = 0;
   const EhtView view = eht_view(f);
-  // the float64 trigonometric invariants of the current state
+  // the float64 trigonometric invariants of the current state, and its ring
+  // group (M::RingPart) in the wave's LDS slot rcache[rci]: a proposal that
+  // moves neither recomputes only the cheap rest of the invariants.  A
+  // proposal that moves the ring group writes its own to the other slot,
+  // which becomes the current one if it is accepted (so no copy of it stays
+  // in registers across the chi^2).
   typename M::Trig tg;
+  int rci = 0;
   {
     float p[M::NP];
     gather_params<M>(pl, p);
     tg = M::trig(p);
+    rcache[0] = M::ring_part(p, tg);
   }
 
   // This iteration's random numbers, substep s on lane s: the coordinate
@@ -488,7 +499,15 @@ __device__ void step_one_chain(const FitDesc& f, int ch, int it, int it_off, flo
       float p[M::NP];
       gather_params<M>(pl, p);
       const typename M::Trig tn = M::trig_uses(t) ? M::trig(p) : tg;
-      const typename M::Inv q = M::prepare(p, tn);
+      const bool ring_moved = M::ring_uses(t);
+      typename M::RingPart rp;
+      if (ring_moved) {
+        rp = M::ring_part(p, tn);
+        rcache[rci ^ 1] = rp;
+      } else {
+        rp = rcache[rci];
+      }
+      const typename M::Inv q = M::assemble(p, tn, rp);
       // one copy of the chi^2 loop per phase form (the branch is per wave)
       c2 = f.hwphase ? eht_chi2_wave<M, false, true>(view, q, ring, bt, ch)
                      : eht_chi2_wave<M, false, false>(view, q, ring, bt, ch);
@@ -498,6 +517,7 @@ __device__ void step_one_chain(const FitDesc& f, int ch, int it, int it_off, flo
         acc = 1;
         chi2c = (float)c2;
         tg = tn;
+        if (ring_moved) rci ^= 1;
         if (lane == k) ++nac;
       } else if (lane == t) {
         pl = old;
@@ -579,12 +599,14 @@ __global__ void __launch_bounds__(64 * WPB) k_step_eht(const FitDesc* fits, cons
                                                        RunClock* clk, TraceBuf tr, int qslot, int it_off) {
   __shared__ float2 ring_all[WPB][RING];
   __shared__ float4 btab[2 * BESSEL_TAB_N];
+  __shared__ typename M::RingPart rcache_all[WPB][2];
   pass_clock(clk);
   if (PERSIST && blockIdx.x == 0 && threadIdx.x < NXCD) clk->qhead[(it_off + 1) & 1][qslot * NXCD + threadIdx.x] = 0;
   load_bessel_tab(btab);
   const BesselTab bt{btab};
   const int wid = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   float2* ring = ring_all[wid];
+  typename M::RingPart* rcache = rcache_all[wid];
   const int it = __builtin_amdgcn_readfirstlane(load_clock(clk, it_off));
   const int xb = (int)(blockIdx.x % NXCD);
   if (!PERSIST) {
@@ -593,7 +615,7 @@ __global__ void __launch_bounds__(64 * WPB) k_step_eht(const FitDesc* fits, cons
     const WorkItem w = work[wv];
     const FitDesc& f = wave_fit(fits, w.fit);
     if (it > last_busy_it(f, it_off)) return;
-    step_one_chain<M>(f, __builtin_amdgcn_readfirstlane(w.idx), it, it_off, ring, bt, tr);
+    step_one_chain<M>(f, __builtin_amdgcn_readfirstlane(w.idx), it, it_off, ring, rcache, bt, tr);
   } else {
     const int hs = it_off & 1;
     for (int r = 0; r < NXCD; ++r) {
@@ -604,7 +626,7 @@ __global__ void __launch_bounds__(64 * WPB) k_step_eht(const FitDesc* fits, cons
         const WorkItem w = work[base + j];
         const FitDesc& f = wave_fit(fits, w.fit);
         if (it <= last_busy_it(f, it_off))
-          step_one_chain<M>(f, __builtin_amdgcn_readfirstlane(w.idx), it, it_off, ring, bt, tr);
+          step_one_chain<M>(f, __builtin_amdgcn_readfirstlane(w.idx), it, it_off, ring, rcache, bt, tr);
       }
     }
   }
@@ -1135,7 +1157,8 @@ static inline size_t occupancy_lds(const StepLaunch& L, int wpb, size_t static_l
 
 template <class M, int WPB, bool PERSIST>
 static void launch_step_form(const StepLaunch& L, const TraceBuf& tr, hipStream_t s) {
-  constexpr size_t static_lds = sizeof(float2) * WPB * RING + sizeof(float4) * 2 * BESSEL_TAB_N;
+  constexpr size_t static_lds =
+      sizeof(float2) * WPB * RING + sizeof(float4) * 2 * BESSEL_TAB_N + sizeof(typename M::RingPart) * 2 * WPB;
   const size_t dyn = occupancy_lds(L, WPB, static_lds);
   const int g = PERSIST ? L.grid : xcd_grid(L.xr, WPB);
   k_step_eht<M, WPB, PERSIST><<<g, 64 * WPB, dyn, s>>>(L.fits, L.work, L.xr, L.clk, tr, L.qslot, L.it_off);
