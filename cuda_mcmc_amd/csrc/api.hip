@@ -223,11 +223,16 @@ int pack_eht(FitHost& h, int idx) {
   h.ntile = (h.nvis_k + TILE - 1) / TILE;
   // one extra tile of zero-weight padding: the kernel prefetches tile t + 1
   h.vis.assign((size_t)(h.ntile + 1) * TILE * 4, 0.0f);
+  // amplitudes as (d sqrt(w), sqrt(w)): the weighted residual (d - |V|) sqrt(w)
+  // is then one FMA per visibility
   for (int i = 0; i < nvis; ++i) {
+    if (!(s.std2r[i] >= 0.0f))
+      FAIL(MHRE_EINVAL, "fit %d: std2r[%d] = %g: amplitude weights must be non-negative", idx, i, (double)s.std2r[i]);
+    const double sw = std::sqrt((double)s.std2r[i]);
     h.vis[4 * i + 0] = s.coor[2 * i];
     h.vis[4 * i + 1] = s.coor[2 * i + 1];
-    h.vis[4 * i + 2] = s.dat[i];
-    h.vis[4 * i + 3] = s.std2r[i];
+    h.vis[4 * i + 2] = (float)((double)s.dat[i] * sw);
+    h.vis[4 * i + 3] = (float)sw;
   }
   std::vector<int> tri((size_t)ncph * 3);
   for (int j = 0; j < ncph; ++j) {

@@ -260,13 +260,15 @@ struct Sgra13 {
 __device__ inline Sgra13::Trig Sgra13::trig(const float* p) {
   const double alpha = p[9], beta = p[10], eta = p[11], th = p[12];
   Trig tg;
+  double sn, cs;
   tg.ceth = cos(eta - th);
-  tg.ca = cos(alpha);
-  tg.sa = sin(alpha);
-  tg.cth = (float)cos(th);
-  tg.sth = (float)sin(th);
-  tg.cb = (float)cos(beta);
-  tg.sb = (float)sin(beta);
+  sincos(alpha, &tg.sa, &tg.ca);   // one argument reduction per angle
+  sincos(th, &sn, &cs);
+  tg.cth = (float)cs;
+  tg.sth = (float)sn;
+  sincos(beta, &sn, &cs);
+  tg.cb = (float)cs;
+  tg.sb = (float)sn;
   return tg;
 }
 

@@ -179,14 +179,14 @@ __device__ double eht_chi2_wave(const EhtView& v, const typename M::Inv& q, floa
     const bool any_large = __float_as_uint(r2cur) > r2_thr;
     const float2 V = M::template vis<BesselTab, HW>(q, d.x, d.y, r2, any_large, bt);
     const float amp = __builtin_amdgcn_sqrtf(fmaf(V.x, V.x, V.y * V.y));
-    const float r = d.z - amp;
-    part = fmaf(r * d.w, r, part);   // zero weight past the amplitude data
+    const float r = fmaf(-amp, d.w, d.z);   // (d - |V|) sqrt(w); zero past the amplitude data
+    part = fmaf(r, r, part);
     if (out) {
       const int i = t * TILE + lane;
       if (i < nvr) {
         const size_t imd = (size_t)ch * v.ndat + i;
         v.datm[imd] = amp;
-        v.chi2m[imd] = r * r * d.w;
+        v.chi2m[imd] = r * r;
         const long long tail = (long long)v.nchain * v.ndat + (long long)ch * nvr + i;
         if (tail < v.ndatm) v.datm[tail] = atan2_fast(V.y, V.x);
       }

@@ -58,8 +58,8 @@ struct FitDesc {
   long long tr_ex;       // dec/chi2 and exchange trace arrays (fits in session order)
 
   // ---- static inputs (device pointers)
-  const float4* vis;       // [(ntile + 1) * TILE] (u, v, amp, 1/sigma^2) for the native
-                           // kernel, zero past nvis
+  const float4* vis;       // [(ntile + 1) * TILE] (u, v, amp / sigma, 1 / sigma) for the
+                           // native kernels (amp sqrt(std2r), sqrt(std2r)), zero past nvis
   const float* rho2;       // [(ntile + 1) * TILE] u^2 + v^2 of each visibility, zero past nvis
   const int4* clo;         // [ncph + TILE] vis indices (i, j, k, datum index) sorted by
                            // ring tile, zero past ncph

@@ -273,9 +273,10 @@ __global__ void __launch_bounds__(256) k_eht_grad9(GradArgs g) {
     const float4 d = gptr(f.vis)[i];
     const Fwd9 fw = sgra9_fwd(q, d.x, d.y, bt);
     const float amp = __builtin_amdgcn_sqrtf(fmaf(fw.Vx, fw.Vx, fw.Vy * fw.Vy));
-    const float r = d.z - amp;
-    chi += (double)(r * r * d.w);
+    const float r = fmaf(-amp, d.w, d.z);   // (d - |V|) sqrt(w): vis holds (d sqrt(w), sqrt(w))
+    chi += (double)(r * r);
     if (g.grad) {
+      // d(r^2)/d|V| = -2 r sqrt(w)
       const float s = -2.0f * r * d.w * __builtin_amdgcn_rcpf(fmaxf(amp, 1e-30f));
       sgra9_bwd(q, fw, s * fw.Vx, s * fw.Vy, a);
     }
@@ -393,7 +394,7 @@ __global__ void __launch_bounds__(256) k_eht_grad9_ring(GradArgs g) {
       const float m2 = fmaxf(fmaf(fw.Vx, fw.Vx, fw.Vy * fw.Vy), 1e-30f);
       const float rs = __builtin_amdgcn_rsqf(m2);   // 1/|V|: |V| = m2 rs, 1/|V|^2 = rs^2
       float sa = 0.0f;
-      if (j < nvr) sa = -2.0f * (d.z - m2 * rs) * d.w * rs;
+      if (j < nvr) sa = -2.0f * fmaf(-m2 * rs, d.w, d.z) * d.w * rs;   // -2 (d - |V|) w / |V|
       // amplitude: s (Vx, Vy); closures: darg(V, S) = S (-Vy, Vx) / |V|^2
       const float k = rings[j & (RING - 1)] * (rs * rs);
       sgra9_bwd(q, fw, fmaf(sa, fw.Vx, -k * fw.Vy), fmaf(sa, fw.Vy, k * fw.Vx), af);
@@ -409,8 +410,8 @@ __global__ void __launch_bounds__(256) k_eht_grad9_ring(GradArgs g) {
       dnext = vis[i + TILE];   // padded by one zero-weight tile
       const Fwd9 fw = sgra9_fwd(q, d.x, d.y, bt);
       if (i < nvr) {
-        const float r = d.z - __builtin_amdgcn_sqrtf(fmaf(fw.Vx, fw.Vx, fw.Vy * fw.Vy));
-        chi += (double)(r * r * d.w);
+        const float r = fmaf(-__builtin_amdgcn_sqrtf(fmaf(fw.Vx, fw.Vx, fw.Vy * fw.Vy)), d.w, d.z);
+        chi += (double)(r * r);
       }
       ringv[i & (RING - 1)] = make_float2(fw.Vx, fw.Vy);
       rings[i & (RING - 1)] = 0.0f;
@@ -529,8 +530,8 @@ __global__ void __launch_bounds__(64) k_eht_gn9(GradArgs g) {
     const float4 d = gptr(f.vis)[i];
     const Fwd9 fw = sgra9_fwd(q, d.x, d.y, bt);
     const float amp = __builtin_amdgcn_sqrtf(fmaf(fw.Vx, fw.Vx, fw.Vy * fw.Vy));
-    const float sw = __builtin_amdgcn_sqrtf(d.w);
-    const float r = (d.z - amp) * sw;
+    const float sw = d.w;   // vis holds (d sqrt(w), sqrt(w))
+    const float r = fmaf(-amp, sw, d.z);
     chi += (double)r * (double)r;
     // dr/dV = -sqrt(w) V / amp
     const float s = -sw * __builtin_amdgcn_rcpf(fmaxf(amp, 1e-30f));

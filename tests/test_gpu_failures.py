@@ -3,6 +3,9 @@
 * An unknown imodel: the reference's model() returns -1 there
   (examples/sgra_model_fit/model.cuh:63) and calc_chi2_terms sums it as data
   (src/gpu_mcmc.cu:301); every entry point here returns MHRE_EMODEL instead.
+* A negative amplitude weight: the native kernels store amplitudes as
+  (d sqrt(w), sqrt(w)), so std2r < 0 (no inverse variance) is MHRE_EINVAL at
+  session creation, and the library works afterwards.
 * The pick redraw: the coordinate pick and the exchange pair pick are redrawn
   when the uniform rounds to n (src/gpu_mcmc.cu:369-372, :549-552).  The keys
   of tests/golden/redraw_keys.json make the first Philox word 0xFFFFFFFF at
@@ -70,6 +73,23 @@ def test_unknown_imodel_is_an_error(imodel, generic, monkeypatch):
     mi.calcmodchi2(d["coor"], d["dat"], d["std2r"], d["icoor"], idat, datm, chi2m, P, chi2c, flag, 1,
                    d["coor"].size, ndat, d["icoor"].size, 3, datm.size, 9, 2, 3)
     assert np.all(np.isfinite(chi2c)) and np.all(chi2c > 0)
+
+
+def test_negative_amplitude_weight_is_an_error():
+    d = _small()
+    std2r = d["std2r"].copy()
+    std2r[3] = -1.0
+    idat = np.array([d["nvis"], 2 * 3 * (d["nvis"] + d["ncph"]), 0], np.int32)
+    f = Fit(synthdata.EHT_PDESCR, synthdata.EHT_PMIN, synthdata.EHT_PMAX, d["coor"], d["dat"], std2r,
+            nbeta=2, nseq=3, nburn=1, niter=1, model="sgra9", imodel=1, icoor=d["icoor"], idat=idat, npass=2)
+    with pytest.raises(_lib.EngineError, match=r"std2r\[3\].*non-negative"):
+        Session([f])
+    g = Fit(synthdata.EHT_PDESCR, synthdata.EHT_PMIN, synthdata.EHT_PMAX, d["coor"], d["dat"], d["std2r"],
+            nbeta=2, nseq=3, nburn=1, niter=1, model="sgra9", imodel=1, icoor=d["icoor"], idat=idat, npass=2)
+    with Session([g]) as s:
+        s.run(2)
+        s.download()
+    assert np.all(np.isfinite(g.chi2c))
 
 
 def _seed(e):
