@@ -21,6 +21,7 @@
 //   k_exchange       replica_exchange (:533-626); one lane per temperature
 //                    slot, 64 / xlanes sequences per wave
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include "mhre_device.h"
 #include "eht_model.h"
 #include "fitdesc.h"
@@ -169,8 +170,11 @@ __device__ double eht_chi2_wave(const EhtView& v, const typename M::Inv& q, floa
   // above q.r.r2thr (RingInv): a compare of the (non-negative) float bits, in
   // scalar registers
   const uint32_t r2_thr = __builtin_amdgcn_readfirstlane(q.r.r2thr);
-  // tile t with its data (d, r2); fetches tile t + 1 into (dn, r2n)
-  auto tile = [&](const int t, const float4& d, const float r2, float4& dn, float& r2n) {
+  // tile t (ring slot J = t % 8, a constant, so the ring store's address is
+  // an instruction offset) with its data (d, r2); fetches tile t + 1 into
+  // (dn, r2n)
+  auto tile = [&](auto J, const int t, const float4& d, const float r2, float4& dn, float& r2n) {
+    constexpr int slot = decltype(J)::value;
     dn = buf_load_f4(vb, l16, (uint32_t)(t + 1) * 1024u);
     r2n = buf_load_f(rb, l4, (uint32_t)(t + 1) * 256u);
     const int2 tin = ldg(v.tinfo, t + 2);
@@ -196,7 +200,7 @@ __device__ double eht_chi2_wave(const EhtView& v, const typename M::Inv& q, floa
       // closure reads (the host checked every closure's window).  Without
       // ring-ordered closures (none, or ring_ok = 0) every tinfo entry names
       // closure 0, so no closure pass runs here.
-      ring_lane[(t & (RING / TILE - 1)) * TILE] = V;
+      ring_lane[slot * TILE] = V;
       const int c1 = ti.x;   // closures whose largest index is in tiles <= t
       if (--kleft == 0 || t + 1 == v.ntile) {
         kleft = v.cgroup;
@@ -230,20 +234,33 @@ __device__ double eht_chi2_wave(const EhtView& v, const typename M::Inv& q, floa
     r2cur = __int_as_float(ti.y);
     ti = tin;
   };
-  // two tiles per trip: the prefetched tile alternates between the a and b
-  // registers, so the prefetch costs no register copies; the fp32 block sum
-  // goes into the fp64 total every fourth trip
+  // eight tiles (one pass round the ring) per trip: the prefetched tile
+  // alternates between the a and b registers, so the prefetch costs no
+  // register copies; the fp32 block sum goes into the fp64 total after each
+  // trip
+  using S0 = std::integral_constant<int, 0>;
+  using S1 = std::integral_constant<int, 1>;
+  using S2 = std::integral_constant<int, 2>;
+  using S3 = std::integral_constant<int, 3>;
+  using S4 = std::integral_constant<int, 4>;
+  using S5 = std::integral_constant<int, 5>;
+  using S6 = std::integral_constant<int, 6>;
+  using S7 = std::integral_constant<int, 7>;
+  static_assert(RING / TILE == 8, "one trip of the tile loop goes once round the ring");
   float4 da = buf_load_f4(vb, l16, 0u), db;
   float ra = buf_load_f(rb, l4, 0u), rbv;
-  for (int t = 0; t < v.ntile; t += 2) {
-    tile(t, da, ra, db, rbv);
-    if (t + 1 < v.ntile) tile(t + 1, db, rbv, da, ra);
-    if ((t & 6) == 6) {
-      acc += (double)part;
-      part = 0.0f;
-    }
+  for (int t = 0; t < v.ntile; t += 8) {
+    tile(S0{}, t, da, ra, db, rbv);
+    if (t + 1 < v.ntile) tile(S1{}, t + 1, db, rbv, da, ra);
+    if (t + 2 < v.ntile) tile(S2{}, t + 2, da, ra, db, rbv);
+    if (t + 3 < v.ntile) tile(S3{}, t + 3, db, rbv, da, ra);
+    if (t + 4 < v.ntile) tile(S4{}, t + 4, da, ra, db, rbv);
+    if (t + 5 < v.ntile) tile(S5{}, t + 5, db, rbv, da, ra);
+    if (t + 6 < v.ntile) tile(S6{}, t + 6, da, ra, db, rbv);
+    if (t + 7 < v.ntile) tile(S7{}, t + 7, db, rbv, da, ra);
+    acc += (double)part;
+    part = 0.0f;
   }
-  acc += (double)part;
   if (v.ncph && !v.ring_ok) {
     // General closure layout: recompute the three model visibilities.
     for (int c = lane; c < v.ncph; c += TILE) {
@@ -375,7 +392,7 @@ __device__ inline SlotAfterSwaps seq_exchange(const FitDesc& f, int iseq, int sl
 // the two sets by the parity of it_off); for it past the fit's last iteration
 // only that exchange runs.
 template <class M>
-__device__ void step_one_chain(const FitDesc& f, int ch, int it, int it_off, float2* ring,
+__device__ __attribute__((always_inline)) void step_one_chain(const FitDesc& f, int ch, int it, int it_off, float2* ring,
                                typename M::RingPart* rcache, const BesselTab& bt, const TraceBuf& tr) {
   const int lane = lane_id();
   const int C = f.nchain, P = f.nprm, N = f.nptot;
