@@ -106,10 +106,11 @@ struct FitHost {
   int nround = 0;
   bool ring_grad = false;        // the gradient may use the single-pass ring kernel
   std::vector<float> qcoef;
+  std::vector<float> qrow;       // single-coordinate update rows (fitdesc.h QROW_*), or empty
   std::vector<int> sumh;
   std::vector<uint32_t> hbits;   // packed acceptance history
   // arena offsets
-  size_t o_vis, o_rho2, o_clo, o_clor, o_clow, o_tinfo, o_qcoef;
+  size_t o_vis, o_rho2, o_clo, o_clor, o_clow, o_tinfo, o_qcoef, o_qrow;
   size_t o_coor, o_dat, o_std2r, o_icoor, o_idat, o_pdescr, o_ivar, o_invar, o_pmint, o_pmaxt, o_beta;
   size_t o_pcur, o_chi2c, o_pstp, o_ncnt, o_nacpt, o_nhist, o_nexch, o_sumh, o_tcur, o_flag, o_ptent, o_ptentn;
   size_t o_pout, o_tout, o_chi2, o_datm, o_chi2m, o_mc;
@@ -128,6 +129,10 @@ int validate(const MhreFitSpec& s, int idx) {
   if (s.nptot > MAXP) FAIL(MHRE_EINVAL, "fit %d: nptot %d exceeds %d", idx, s.nptot, MAXP);
   if (s.nadj <= 0) FAIL(MHRE_EINVAL, "fit %d: nadj must be positive", idx);
   if (s.nburn < 0 || s.niter < 0) FAIL(MHRE_EINVAL, "fit %d: negative nburn/niter", idx);
+  // the step kernels address the acceptance history (and the quadratic-system
+  // steps also pstp, n_cnt, sumh and n_acpt) with 32-bit element offsets
+  if ((long long)s.nbeta * s.nseq * s.nprm * ((s.nadj + 31) / 32) >= (1ll << 31))
+    FAIL(MHRE_EINVAL, "fit %d: nbeta * nseq * nprm * ceil(nadj / 32) must stay below 2^31", idx);
   if (!s.pdescr || !s.ivar || !s.invar || !s.pmint || !s.pmaxt || !s.beta || !s.pcur || !s.chi2c ||
       !s.pstp || !s.n_cnt || !s.n_acpt || !s.n_hist || !s.n_exch || !s.tcur || !s.flag || !s.ptent ||
       !s.ptentn || !s.dat || !s.std2r)
@@ -342,6 +347,29 @@ int pack_quad(FitHost& h, int idx) {
     c[ns + n] = s.dat[e];
     c[ns + n + 1] = s.std2r[e];
   }
+  // Rows of the single-coordinate update (k_iter_quad: ladders of at most 64
+  // rungs) for systems of at most QROW_MAXEQ equations with non-negative
+  // weights (the rows carry sqrt(w)); MHRE_QUAD_FULL=1 keeps the evaluation
+  // of every equation in full.
+  h.qrow.clear();
+  bool rows = s.nbeta <= 64 && neq <= QROW_MAXEQ && !env_flag("MHRE_QUAD_FULL");
+  for (int e = 0; rows && e < neq; ++e) rows = s.std2r[e] >= 0.0f && std::isfinite(s.std2r[e]);
+  if (rows) {
+    const int ts = QROW_TS(n), E = QROW_MAXEQ;
+    h.qrow.assign((size_t)QROW_FLOATS(n), 0.0f);
+    for (int e = 0; e < neq; ++e) {
+      const float* Q = s.coor + (size_t)e * n * n;
+      const float* b = s.coor + (size_t)neq * n * n + (size_t)e * n;
+      const float sw = std::sqrt(s.std2r[e]);
+      for (int t = 0; t < n; ++t) {
+        float* blk = &h.qrow[(size_t)t * ts];
+        for (int j = 0; j < n; ++j) blk[j * E + e] = sw * (j == t ? Q[t * n + t] : Q[t * n + j] + Q[j * n + t]);
+        blk[n * E + e] = sw * b[t];
+        blk[(n + 1) * E + e] = sw * Q[t * n + t];
+      }
+      h.qrow[(size_t)n * ts + e] = sw;
+    }
+  }
   return MHRE_OK;
 }
 
@@ -410,25 +438,34 @@ constexpr int MAX_GROUPS = 8;
 // n_hist[a][q] (0/1, q = k * nchain + chain) -> bit a % 32 of word [a / 32][q]
 // Packs the 0/1 history n_hist[nadj][npc] into bits and, in the same pass,
 // the running accept sums sum[q] = number of set entries of q.
-void pack_hist(const int* n_hist, int nadj, size_t npc, std::vector<uint32_t>& bits, std::vector<int>& sum) {
-  bits.assign((size_t)hist_words(nadj) * npc, 0u);
+// n_hist[a][k][chain] (0/1) <-> bit a % 32 of hbits[chain][a / 32][k]
+// (fitdesc.h), with sum[k][chain] = the set entries of (k, chain).
+void pack_hist(const int* n_hist, int nadj, int P, int C, std::vector<uint32_t>& bits, std::vector<int>& sum) {
+  const int W = hist_words(nadj);
+  const size_t npc = (size_t)P * C;
+  bits.assign((size_t)W * npc, 0u);
   sum.assign(npc, 0);
   for (int a = 0; a < nadj; ++a) {
     const int* h = n_hist + (size_t)a * npc;
-    uint32_t* w = bits.data() + (size_t)(a >> 5) * npc;
     const uint32_t bit = 1u << (a & 31);
-    for (size_t q = 0; q < npc; ++q) {
-      const int on = h[q] != 0;
-      w[q] |= on ? bit : 0u;
-      sum[q] += on;
-    }
+    for (int k = 0; k < P; ++k)
+      for (int c = 0; c < C; ++c) {
+        const size_t q = (size_t)k * C + c;
+        const int on = h[q] != 0;
+        bits[((size_t)c * W + (a >> 5)) * P + k] |= on ? bit : 0u;
+        sum[q] += on;
+      }
   }
 }
 
-void unpack_hist(const std::vector<uint32_t>& bits, int nadj, size_t npc, int* n_hist) {
+void unpack_hist(const std::vector<uint32_t>& bits, int nadj, int P, int C, int* n_hist) {
+  const int W = hist_words(nadj);
+  const size_t npc = (size_t)P * C;
   for (int a = 0; a < nadj; ++a)
-    for (size_t q = 0; q < npc; ++q)
-      n_hist[(size_t)a * npc + q] = (int)((bits[(size_t)(a >> 5) * npc + q] >> (a & 31)) & 1u);
+    for (int k = 0; k < P; ++k)
+      for (int c = 0; c < C; ++c)
+        n_hist[(size_t)a * npc + (size_t)k * C + c] =
+            (int)((bits[((size_t)c * W + (a >> 5)) * P + k] >> (a & 31)) & 1u);
 }
 
 // ------------------------------------------------------------- session
@@ -437,6 +474,7 @@ struct KindWork {
   int kind;
   int quad_n;
   bool fused = false;   // quadratic systems: step and exchange in one launch, items (fit, first sequence)
+  bool rows = false;    // fused quadratic systems: every fit has single-coordinate update rows
   std::vector<WorkItem> step, eval;
   XcdRanges xr{};   // wave-per-chain kinds: step list dealt into per-XCD ranges
   WorkItem* d_step = nullptr;
@@ -701,7 +739,7 @@ int launch_steps(Session* S, const Group& g, int i) {
     hipError_t e = hipSuccess;
     if (k.kind == K_SGRA9) e = launch_step_eht(MODEL_SGRA9, L, g.stream);
     else if (k.kind == K_SGRA13) e = launch_step_eht(MODEL_SGRA13, L, g.stream);
-    else if (k.kind == K_QUAD && k.fused) e = launch_iter_quad(k.quad_n, L, g.stream);
+    else if (k.kind == K_QUAD && k.fused) e = launch_iter_quad(k.quad_n, k.rows, L, g.stream);
     else if (k.kind == K_QUAD) e = launch_step_quad(k.quad_n, L, g.stream);
     else e = launch_step_user(L, g.stream);
     HIPCHK(e);
@@ -934,6 +972,7 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
     h.o_clow = A.reserve<float>(h.clow.size());
     h.o_tinfo = A.reserve<int>(h.tinfo.size());
     h.o_qcoef = A.reserve<float>(h.qcoef.size());
+    h.o_qrow = A.reserve<float>(h.qrow.size());
     h.o_coor = A.reserve<float>(s.coor ? s.ncoor : 0);
     h.o_dat = A.reserve<float>(s.ndat);
     h.o_std2r = A.reserve<float>(s.ndat);
@@ -1026,11 +1065,13 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
     d.key0 = (uint32_t)(s.seed & 0xffffffffull);
     d.key1 = (uint32_t)(s.seed >> 32);
     d.adapt_target = s.adapt_target > 0.0f ? s.adapt_target : 0.30f;
+    rate_threshold(d.adapt_target, &d.adapt_mid, &d.adapt_tie);
     d.quad_n = h.quad_n; d.quad_neq = s.ndat;
     d.record = h.record ? 1 : 0;
     d.vis = at<float4>(S, h.o_vis); d.rho2 = at<float>(S, h.o_rho2);
     d.clo = at<int4>(S, h.o_clo); d.clor = at<int4>(S, h.o_clor); d.clow = at<float4>(S, h.o_clow);
     d.tinfo = at<int2>(S, h.o_tinfo); d.qcoef = at<float>(S, h.o_qcoef);
+    d.qrow = h.qrow.empty() ? nullptr : at<float>(S, h.o_qrow);
     d.coor = at<float>(S, h.o_coor); d.dat = at<float>(S, h.o_dat); d.std2r = at<float>(S, h.o_std2r);
     d.icoor = at<int>(S, h.o_icoor); d.idat = at<int>(S, h.o_idat);
     d.pdescr = at<int>(S, h.o_pdescr); d.ivar = at<int>(S, h.o_ivar); d.invar = at<int>(S, h.o_invar);
@@ -1063,6 +1104,7 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
     U.add(h.o_clow, h.clow.data(), h.clow.size());
     U.add(h.o_tinfo, h.tinfo.data(), h.tinfo.size());
     U.add(h.o_qcoef, h.qcoef.data(), h.qcoef.size());
+    U.add(h.o_qrow, h.qrow.data(), h.qrow.size());
     U.add(h.o_coor, s.coor, s.coor ? s.ncoor : 0);
     U.add(h.o_dat, s.dat, s.ndat);
     U.add(h.o_std2r, s.std2r, s.ndat);
@@ -1081,7 +1123,7 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
     U.add(h.o_nacpt, s.n_acpt, P * C);
     // the history as bits and the running accept sums over its nadj entries
     // (the device keeps sumh[k][chain] equal to the set bits of the chain's ring)
-    pack_hist(s.n_hist, s.nadj, P * C, h.hbits, h.sumh);
+    pack_hist(s.n_hist, s.nadj, P, C, h.hbits, h.sumh);
     U.add(h.o_nhist, h.hbits.data(), h.hbits.size());
     U.add(h.o_nexch, s.n_exch, C);
     U.add(h.o_tcur, s.tcur, C);
@@ -1194,18 +1236,22 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
         // one kernel per compiled system size; ladders of at most 64 rungs
         // run step and exchange in one launch (one item per wave: the fit
         // and its first sequence), longer ones lane per chain
+        // (fused fits with update rows and those without take kernels of
+        // their own: the rows form holds every equation's residual in registers)
         for (int n : {2, 4, 8}) {
-          KindWork kf, kw;
-          kf.kind = kw.kind = kk;
-          kf.quad_n = kw.quad_n = n;
-          kf.fused = true;
+          KindWork kr, kf, kw;
+          kr.kind = kf.kind = kw.kind = kk;
+          kr.quad_n = kf.quad_n = kw.quad_n = n;
+          kr.fused = kf.fused = true;
+          kr.rows = true;
           for (int i : idx) {
             if (S->fits[i].quad_n != n) continue;
             if (quad_fused(S->fits[i])) {
               const int per = 64 / exchange_lanes(S->fits[i].s.nbeta);
               int s0, s1;
               seq_range(i, g, &s0, &s1);
-              for (int q = s0; q < s1; q += per) kf.step.push_back({i, q});
+              KindWork& k = S->fits[i].qrow.empty() ? kf : kr;
+              for (int q = s0; q < s1; q += per) k.step.push_back({i, q});
               continue;
             }
             const size_t before = kw.step.size();
@@ -1213,6 +1259,7 @@ int create(const MhreFitSpec* specs, int nfits, const MhreOptions* opt, Session*
             if (kw.step.size() > before)
               while (kw.step.size() % 64) kw.step.push_back({i, -1});
           }
+          if (!kr.step.empty()) grp.kinds.push_back(std::move(kr));
           if (!kf.step.empty()) grp.kinds.push_back(std::move(kf));
           if (!kw.step.empty()) grp.kinds.push_back(std::move(kw));
         }
@@ -1399,7 +1446,7 @@ int download(Session* S, int model_data) {
   }
   CHK(D.flush(S));
   for (FitHost& h : S->fits)
-    unpack_hist(h.hbits, h.s.nadj, (size_t)h.s.nprm * h.nchain, h.s.n_hist);
+    unpack_hist(h.hbits, h.s.nadj, h.s.nprm, h.nchain, h.s.n_hist);
   if (model_data) {
     // model data and chi^2 terms of each chain's final state; chi2c is
     // recomputed identically and already copied out above
@@ -1801,6 +1848,7 @@ int mhre_session_fit_info(const void* session, int fit, int* info, int ninfo) {
   info[1] = (h.xfused || quad_fused(h)) ? 1 : 0;
   info[2] = h.hwphase ? 1 : 0;
   info[3] = (int)S->groups.size();
+  if (ninfo >= 5) info[4] = h.qrow.empty() ? 0 : 1;
   return MHRE_OK;
 }
 

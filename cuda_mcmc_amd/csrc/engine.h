@@ -76,8 +76,9 @@ hipError_t launch_step_eht(int model, const StepLaunch& L, hipStream_t s);
 hipError_t launch_eval_quad(int n, const FitDesc* fits, const WorkItem* work, int nwork, hipStream_t s);
 hipError_t launch_step_quad(int n, const StepLaunch& L, hipStream_t s);
 // step + exchange of whole sequences (ladders of at most 64 rungs); work items
-// are (fit, first sequence of the wave)
-hipError_t launch_iter_quad(int n, const StepLaunch& L, hipStream_t s);
+// are (fit, first sequence of the wave); rows: every fit of the launch has
+// single-coordinate update rows (FitDesc::qrow)
+hipError_t launch_iter_quad(int n, bool rows, const StepLaunch& L, hipStream_t s);
 // Exchange of the sequences in xwork (items: fit, first sequence of the wave):
 // per iteration for fits without xfused (L.close_k = 0); the closing launch of
 // a segment (L.close_k = K) finishes the xfused fits' last exchange, leaves

@@ -467,7 +467,7 @@ __device__ __attribute__((always_inline)) void step_one_chain(const FitDesc& f, 
   uint32_t hword = 0u;
   if (burn && lane < P) {
     hw_idx = (cnt % f.nadj) >> 5;
-    hword = f.hbits[((size_t)hw_idx * P + lane) * C + ch];
+    hword = f.hbits[(ch * f.hwords + hw_idx) * P + lane];
   }
   int flag = 0, tlast = 0;
   float told = 0.0f;
@@ -555,9 +555,9 @@ __device__ __attribute__((always_inline)) void step_one_chain(const FitDesc& f, 
       ++cnt;
       const int w2 = (cnt % f.nadj) >> 5;
       if (w2 != hw_idx) {
-        f.hbits[((size_t)hw_idx * P + lane) * C + ch] = hword;
+        f.hbits[(ch * f.hwords + hw_idx) * P + lane] = hword;
         hw_idx = w2;
-        hword = f.hbits[((size_t)hw_idx * P + lane) * C + ch];
+        hword = f.hbits[(ch * f.hwords + hw_idx) * P + lane];
       }
     }
     if (tr.dec && it >= tr.it0 && it < tr.it0 + tr.nit && lane == 0) {
@@ -574,7 +574,7 @@ __device__ __attribute__((always_inline)) void step_one_chain(const FitDesc& f, 
     f.n_acpt[(size_t)lane * C + ch] = nac;
     if (burn) {
       f.sumh[(size_t)lane * C + ch] = sh;
-      f.hbits[((size_t)hw_idx * P + lane) * C + ch] = hword;
+      f.hbits[(ch * f.hwords + hw_idx) * P + lane] = hword;
     }
   }
   if (lane == 0) {
@@ -664,23 +664,82 @@ struct QuadSys {
   // and no table of the NS products x_i x_j, so a lane holds little more
   // than x (the NQ inner sums are independent chains).
   template <class CP>
+  __device__ static float lhs(CP c, const float (&x)[NQ]) {
+    float l = 0.0f;
+    int m = 0;
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+      float t = c[NS + i];
+#pragma unroll
+      for (int j = i; j < NQ; ++j) t = fmaf(c[m++], x[j], t);
+      l = fmaf(x[i], t, l);
+    }
+    return l;
+  }
+  template <class CP>
   __device__ static double chi2(CP qc, int neq, const float (&x)[NQ]) {
     double acc = 0.0;
     for (int e = 0; e < neq; ++e) {
       const CP c = qc + (size_t)e * QS;
-      float lhs = 0.0f;
+      float l = 0.0f;
       int m = 0;
 #pragma unroll
       for (int i = 0; i < NQ; ++i) {
         float t = c[NS + i];
 #pragma unroll
         for (int j = i; j < NQ; ++j) t = fmaf(c[m++], x[j], t);
-        lhs = fmaf(x[i], t, lhs);
+        l = fmaf(x[i], t, l);
       }
-      const float r = c[NS + NQ] - lhs;
+      const float r = c[NS + NQ] - l;
       acc += (double)(r * r * c[NS + NQ + 1]);
     }
     return acc;
+  }
+  // Weighted residuals sqrt(w_e) (lhs_e - rhs_e) of the first neq equations
+  // (at most QROW_MAXEQ), sw = the row table's sqrt(w_e).
+  template <class CP>
+  __device__ static void residuals(CP qc, const CONSTANT float* sw, int neq, const float (&x)[NQ],
+                                   float (&r)[QROW_MAXEQ]) {
+#pragma unroll
+    for (int e = 0; e < QROW_MAXEQ; ++e) {
+      r[e] = 0.0f;
+      if (e < neq) {
+        const CP c = qc + (size_t)e * QS;
+        r[e] = (lhs(c, x) - c[NS + NQ]) * sw[e];
+      }
+    }
+  }
+  // The residuals after x_t moves from x[t] to xn (d = xn - x[t]), from the
+  // chain's LDS block blk = rows + t * QROW_TS (every equation side by side;
+  // rows past the fit's equations are zero, as are their residuals); returns
+  // their sum of squares (fp32: at most QROW_MAXEQ terms).
+  __device__ static float moved(const float* blk, const float (&x)[NQ], float xn, float d,
+                                const float (&r)[QROW_MAXEQ], float (&rn)[QROW_MAXEQ]) {
+    static_assert(QROW_MAXEQ == 8, "two float4 per term");
+    const float4* q = reinterpret_cast<const float4*>(blk);
+    float g[8];
+    {
+      const float4 b0 = q[2 * NQ], b1 = q[2 * NQ + 1];
+      g[0] = b0.x; g[1] = b0.y; g[2] = b0.z; g[3] = b0.w;
+      g[4] = b1.x; g[5] = b1.y; g[6] = b1.z; g[7] = b1.w;
+    }
+#pragma unroll
+    for (int j = 0; j <= NQ; ++j) {
+      const float v = j < NQ ? x[j] : xn;
+      const int k = j < NQ ? 2 * j : 2 * NQ + 2;   // term n + 1: a_tt times the new value
+      const float4 c0 = q[k], c1 = q[k + 1];
+      g[0] = fmaf(c0.x, v, g[0]); g[1] = fmaf(c0.y, v, g[1]);
+      g[2] = fmaf(c0.z, v, g[2]); g[3] = fmaf(c0.w, v, g[3]);
+      g[4] = fmaf(c1.x, v, g[4]); g[5] = fmaf(c1.y, v, g[5]);
+      g[6] = fmaf(c1.z, v, g[6]); g[7] = fmaf(c1.w, v, g[7]);
+    }
+    float a = 0.0f;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      rn[e] = fmaf(d, g[e], r[e]);
+      a = fmaf(rn[e], rn[e], a);
+    }
+    return a;
   }
 };
 
@@ -701,10 +760,19 @@ __global__ void __launch_bounds__(256) k_eval_quad(const FitDesc* fits, const Wo
 // CACHE: in the burn-in stage the adaptation counters n_cnt and sumh of all
 // the chain's parameters are read once (one coalesced row per parameter) and
 // kept in registers for the iteration, instead of one scattered
-// read-modify-write per proposal (lanes pick different parameters).
-template <int NQ, bool CACHE>
+// read-modify-write per proposal (lanes pick different parameters); the
+// step sizes and acceptance counts of all its parameters sit in the wave's
+// LDS for the iteration (stage: [NQ][64] floats then [NQ][64] ints, column
+// = lane), read and written back once.  Scattered single-word accesses per
+// proposal had made the launch HBM-bound (every lane a cache line of its own).
+// ROWS: the weighted residual of every equation is evaluated in full once,
+// then kept in registers; a proposal, which moves one coordinate, updates
+// them from its block of the fit's row table in LDS (QROW_* layout) with
+// n + 3 operations per equation instead of n(n+1)/2 + n.  Its chi^2 is the
+// same sum of squares to float rounding (sqrt(w) r)^2 against r^2 w.
+template <int NQ, bool CACHE, bool ROWS = false>
 __device__ unsigned quad_substeps(const FitDesc& f, int ch, int it, const TraceBuf& tr, float (&x)[NQ],
-                                  float& chi2c) {
+                                  float& chi2c, const float* rows = nullptr, float* stage = nullptr) {
   const int C = f.nchain, P = f.nprm;
   const CONSTANT float* qc = wave_const(f.qcoef);
   const int neq = __builtin_amdgcn_readfirstlane(f.quad_neq);
@@ -720,6 +788,9 @@ __device__ unsigned quad_substeps(const FitDesc& f, int ch, int it, const TraceB
   const int ibeta = ch / f.nseq;
   const bool burn = it < f.nburn;
   const double beta = (double)gptr(f.beta)[ibeta];
+  const int nadj = __builtin_amdgcn_readfirstlane(f.nadj);
+  const int hwn = __builtin_amdgcn_readfirstlane(f.hwords);
+  const float rnadj = 1.0f / (float)nadj;
   int cnt_c[NQ], sh_c[NQ];
   if (CACHE && burn) {
 #pragma unroll
@@ -728,6 +799,20 @@ __device__ unsigned quad_substeps(const FitDesc& f, int ch, int it, const TraceB
       sh_c[i] = i < P ? sumh[(size_t)i * C + ch] : 0;
     }
   }
+  const int ln = lane_id();
+  float* sp = stage;
+  int* sa = reinterpret_cast<int*>(stage + NQ * 64);
+  if (CACHE) {
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+      if (i < P) {
+        sp[i * 64 + ln] = pstp[i * C + ch];
+        sa[i * 64 + ln] = n_acpt[i * C + ch];
+      }
+    }
+  }
+  float res[QROW_MAXEQ], resn[QROW_MAXEQ];
+  if (ROWS) QuadSys<NQ>::residuals(qc, wave_const(f.qrow) + NQ * QROW_TS(NQ), neq, x, res);
   int flag = 0, tlast = 0;
   float told = 0.0f;
   unsigned nev = 0;
@@ -735,11 +820,30 @@ __device__ unsigned quad_substeps(const FitDesc& f, int ch, int it, const TraceB
     const U4 r = philox4x32_10((uint32_t)ch, (uint32_t)it, (uint32_t)s, 0u, f.key0, f.key1);
     const int k = pick_with_retry(r.x, P, (uint32_t)ch, (uint32_t)it, (uint32_t)s, 0u, f.key0, f.key1);
     const int t = ivar[k];
-    const size_t pk = (size_t)k * C + ch;
+    const int pk = k * C + ch;   // 32-bit element offsets: the host bounds nprm * nchain * hwords
     float old = 0.0f;
 #pragma unroll
     for (int i = 0; i < NQ; ++i) old = (i == t) ? x[i] : old;
-    const float sd = pstp[pk];
+    const float sd = CACHE ? sp[k * 64 + ln] : pstp[pk];
+    // burn-in: the acceptance-history word of this outcome's entry
+    // cnt % nadj (the oldest one), read before the proposal so that its
+    // latency overlaps the chi^2
+    int cnt = 0, sh0 = 0, ha = 0;
+    uint32_t hword = 0;
+    if (burn) {
+      if (CACHE) {
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+          cnt = (i == k) ? cnt_c[i] : cnt;
+          sh0 = (i == k) ? sh_c[i] : sh0;
+        }
+      } else {
+        cnt = n_cnt[pk];
+        sh0 = sumh[pk];
+      }
+      ha = mod_small(cnt, nadj, rnadj);
+      hword = hbits[(ch * hwn + (ha >> 5)) * P + k];
+    }
     // The Gaussian step in fp32 with the hardware log2 and cosine (which takes
     // revolutions, so cos(2 pi u2) is one instruction; its absolute error is
     // below 2e-7): z is within about 2e-6 of the float64 Box-Muller value at
@@ -753,17 +857,28 @@ __device__ unsigned quad_substeps(const FitDesc& f, int ch, int it, const TraceB
     int acc = 0;
     double c2 = 0.0;
     if (inprior) {
+      if (ROWS) {
+        c2 = (double)QuadSys<NQ>::moved(rows + t * QROW_TS(NQ), x, pn, pn - old, res, resn);
+      } else {
 #pragma unroll
-      for (int i = 0; i < NQ; ++i) x[i] = (i == t) ? pn : x[i];
-      c2 = QuadSys<NQ>::chi2(qc, neq, x);
+        for (int i = 0; i < NQ; ++i) x[i] = (i == t) ? pn : x[i];
+        c2 = QuadSys<NQ>::chi2(qc, neq, x);
+      }
       ++nev;
       const double xx = -0.5 * beta * (c2 - (double)chi2c);
       // log u in fp32 (hardware log2): its error, ~1e-7 |log u|, is far inside the tie band
       if ((double)(0.69314718055994531f * __builtin_amdgcn_logf(u01(r.w))) <= xx) {
         acc = 1;
         chi2c = (float)c2;
-        n_acpt[pk] += 1;
-      } else {
+        if (CACHE) sa[k * 64 + ln] += 1;
+        else n_acpt[pk] += 1;
+        if (ROWS) {
+#pragma unroll
+          for (int i = 0; i < NQ; ++i) x[i] = (i == t) ? pn : x[i];
+#pragma unroll
+          for (int e = 0; e < QROW_MAXEQ; ++e) res[e] = resn[e];
+        }
+      } else if (!ROWS) {
 #pragma unroll
         for (int i = 0; i < NQ; ++i) x[i] = (i == t) ? old : x[i];
       }
@@ -772,26 +887,14 @@ __device__ unsigned quad_substeps(const FitDesc& f, int ch, int it, const TraceB
     told = old;
     tlast = t;
     if (burn) {
-      int cnt = 0, sh0 = 0;
-      if (CACHE) {
-#pragma unroll
-        for (int i = 0; i < NQ; ++i) {
-          cnt = (i == k) ? cnt_c[i] : cnt;
-          sh0 = (i == k) ? sh_c[i] : sh0;
-        }
-      } else {
-        cnt = n_cnt[pk];
-        sh0 = sumh[pk];
-      }
-      const int a = cnt % f.nadj;   // oldest history entry, replaced by this outcome
-      GLOBAL uint32_t* hw = hbits + ((size_t)(a >> 5) * P + k) * C + ch;
-      const uint32_t word = *hw, bit = 1u << (a & 31);
-      const int hold = (word & bit) ? 1 : 0;
-      *hw = acc ? (word | bit) : (word & ~bit);
+      const uint32_t bit = 1u << (ha & 31);
+      const int hold = (hword & bit) ? 1 : 0;
+      hbits[(ch * hwn + (ha >> 5)) * P + k] = acc ? (hword | bit) : (hword & ~bit);
       const int sh = sh0 + acc - hold;
-      const int den = cnt < f.nadj ? cnt : f.nadj;
-      const float rate = (float)sh / (float)den;
-      pstp[pk] = rate < f.adapt_target ? (float)((double)sd / 1.01) : (float)((double)sd * 1.01);
+      const int den = cnt < nadj ? cnt : nadj;
+      const float nsd = rate_below(sh, den, f.adapt_mid, f.adapt_tie) ? step_shrink(sd) : step_grow(sd);
+      if (CACHE) sp[k * 64 + ln] = nsd;
+      else pstp[pk] = nsd;
       if (CACHE) {
 #pragma unroll
         for (int i = 0; i < NQ; ++i) {
@@ -815,6 +918,15 @@ __device__ unsigned quad_substeps(const FitDesc& f, int ch, int it, const TraceB
       if (i < P) {
         n_cnt[(size_t)i * C + ch] = cnt_c[i];
         sumh[(size_t)i * C + ch] = sh_c[i];
+      }
+    }
+  }
+  if (CACHE) {
+#pragma unroll
+    for (int i = 0; i < NQ; ++i) {
+      if (i < P) {
+        n_acpt[i * C + ch] = sa[i * 64 + ln];
+        if (burn) pstp[i * C + ch] = sp[i * 64 + ln];
       }
     }
   }
@@ -1033,9 +1145,12 @@ __global__ void __launch_bounds__(64) k_exchange(const FitDesc* fits, const Work
 // gpu_mcmc.cu:361-526 then :533-626), so the chains are the same bit for bit.
 // The burn-in counters held in registers (quad_substeps<NQ, true>) take the
 // register cap to 7 waves per SIMD.
-// The body of k_iter_quad for the wave's G sequences from iseq0.
-template <int NQ>
-__device__ void iter_quad_wave(const FitDesc& f, int iseq0, int it, const TraceBuf& tr) {
+// The body of k_iter_quad for the wave's G sequences from iseq0; stage is
+// the wave's LDS for its chains' step sizes and acceptance counts
+// (2 * NQ * 64 words), and with ROWS (fits that have update rows) qlds the
+// wave's LDS for the rows (QROW_LDS floats).
+template <int NQ, bool ROWS>
+__device__ void iter_quad_wave(const FitDesc& f, int iseq0, int it, const TraceBuf& tr, float* qlds, float* stage) {
   const int lane = lane_id();
   const int B = f.nbeta, S = f.nseq, W = f.xlanes;
   const int lg = 6 - __builtin_ctz((unsigned)W);   // log2 G
@@ -1058,7 +1173,15 @@ __device__ void iter_quad_wave(const FitDesc& f, int iseq0, int it, const TraceB
     c2 = chi2c_a[ch];
     tc = tcur_a[ch];
   }
-  const unsigned nev = slot ? quad_substeps<NQ, true>(f, ch, it, tr, x, c2) : 0u;
+  if (ROWS) {
+    // the fit's update rows into this wave's LDS, every lane taking part
+    constexpr int n4 = NQ * QROW_TS(NQ) / 4;
+    const GLOBAL float4* src = (const GLOBAL float4*)gptr(f.qrow);
+    float4* dst = reinterpret_cast<float4*>(qlds);
+    for (int i = lane; i < n4; i += 64) dst[i] = src[i];
+    ring_fence();
+  }
+  const unsigned nev = slot ? quad_substeps<NQ, true, ROWS>(f, ch, it, tr, x, c2, qlds, stage) : 0u;
   wave_count(f, nev, slot ? (unsigned)f.nprm : 0u);
 
   // exchange: as exchange_wave, with the partner of slot k at lane k * G + g
@@ -1115,8 +1238,12 @@ __device__ void iter_quad_wave(const FitDesc& f, int iseq0, int it, const TraceB
   }
 }
 
-template <int NQ>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7)))
+// The rows form keeps every equation's residual in registers; its LDS
+// (rows and stage, 27 KB per work-group at NQ = 8) already limits it to 5
+// waves per SIMD, which is also its register cap.  The full form takes 16 KB
+// per work-group and the register cap of 6 waves (at 7 it spills).
+template <int NQ, bool ROWS>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ROWS ? 5 : 6)))
 k_iter_quad(const FitDesc* fits, const WorkItem* work, int nwork, RunClock* clk, TraceBuf tr, int it_off) {
   const int wv = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
   if (wv >= nwork) return;
@@ -1124,7 +1251,11 @@ k_iter_quad(const FitDesc* fits, const WorkItem* work, int nwork, RunClock* clk,
   const WorkItem w = work[wv];
   const FitDesc& f = wave_fit(fits, w.fit);
   const int it = __builtin_amdgcn_readfirstlane(load_clock(clk, it_off));
-  if (it < f.nburn + f.niter) iter_quad_wave<NQ>(f, w.idx, it, tr);
+  // 256-thread work-groups: 4 waves
+  __shared__ __align__(16) float qlds[ROWS ? 4 : 1][ROWS ? QROW_LDS : 4];
+  __shared__ float qstage[4][2 * NQ * 64];
+  if (it < f.nburn + f.niter)
+    iter_quad_wave<NQ, ROWS>(f, w.idx, it, tr, ROWS ? qlds[threadIdx.x >> 6] : nullptr, qstage[threadIdx.x >> 6]);
 }
 
 // ---------------------------------------------------------------- launchers
@@ -1230,17 +1361,22 @@ hipError_t launch_step_quad(int n, const StepLaunch& L, hipStream_t s) {
   return hipGetLastError();
 }
 
-hipError_t launch_iter_quad(int n, const StepLaunch& L, hipStream_t s) {
-  if (L.nwork <= 0) return hipSuccess;
+template <bool ROWS>
+static hipError_t launch_iter_quad_form(int n, const StepLaunch& L, hipStream_t s) {
   const TraceBuf tr{L.tr_dec, L.tr_chi2, L.tr_ex, L.tr_it0, L.tr_nit};
   const int g = cdiv(L.nwork, 4);   // one work item per wave, 4 waves per work-group
   switch (n) {
-    case 2: k_iter_quad<2><<<g, 256, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr, L.it_off); break;
-    case 4: k_iter_quad<4><<<g, 256, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr, L.it_off); break;
-    case 8: k_iter_quad<8><<<g, 256, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr, L.it_off); break;
+    case 2: k_iter_quad<2, ROWS><<<g, 256, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr, L.it_off); break;
+    case 4: k_iter_quad<4, ROWS><<<g, 256, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr, L.it_off); break;
+    case 8: k_iter_quad<8, ROWS><<<g, 256, 0, s>>>(L.fits, L.work, L.nwork, L.clk, tr, L.it_off); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
+}
+
+hipError_t launch_iter_quad(int n, bool rows, const StepLaunch& L, hipStream_t s) {
+  if (L.nwork <= 0) return hipSuccess;
+  return rows ? launch_iter_quad_form<true>(n, L, s) : launch_iter_quad_form<false>(n, L, s);
 }
 
 // A closing launch with no sequence to exchange still runs one work-group, to

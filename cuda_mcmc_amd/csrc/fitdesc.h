@@ -9,8 +9,10 @@
 //   tout[nbeta][nseq][niter], chi2[nseq][niter].
 // Chain index ist = nseq * ibeta + iseq (src/gpu_mcmc.cu:247).
 // The acceptance history n_hist[nadj][nprm][nbeta][nseq] (0/1 entries) is
-// kept as bits: hbits[hwords][nprm][nchain], entry a of (k, chain) is bit
-// a % 32 of word [a / 32][k][chain]; it is packed and unpacked on the host.
+// kept as bits, chain-major: hbits[nchain][hwords][nprm], entry a of
+// (k, chain) is bit a % 32 of word [chain][a / 32][k], so a chain's whole
+// history (hwords * nprm words, 128 bytes for 8 parameters and nadj 100)
+// shares its cache lines; it is packed and unpacked on the host.
 #pragma once
 #include <stdint.h>
 
@@ -30,6 +32,21 @@ constexpr int RING = 512;      // per-wave LDS ring of model visibilities
 constexpr int MAX_CGROUP = 4;  // most tiles whose closures are formed together
 constexpr int TILE = 64;       // visibilities per wave step
 
+// Quadratic systems, single-coordinate update: moving x_t by d changes the
+// left side of equation e by d (b_t + sum_j a_tj x_j + a_tt x'_t), x'_t the
+// new value, where a_tj is the coefficient of x_t x_j (Q_tj + Q_jt for j != t,
+// Q_tt for j = t).  Block t of the table holds, for term j = 0 .. n + 1, the
+// QROW_MAXEQ values sqrt(w_e) a_tj (j < n), sqrt(w_e) b_t (j = n) and
+// sqrt(w_e) a_tt (j = n + 1) of equations e = 0 .. QROW_MAXEQ - 1 (zero past
+// the fit's equations), so one lane updates all equations side by side with
+// two 16-byte LDS reads per term.  Blocks lie QROW_TS(n) floats apart, an odd
+// multiple of 16 bytes, so the n blocks that the lanes of a wave read at once
+// fall in disjoint LDS banks.  Then sqrt(w_e) per equation.
+constexpr int QROW_MAXEQ = 8;
+constexpr int QROW_TS(int n) { return (n + 2) * QROW_MAXEQ + 4; }
+constexpr int QROW_FLOATS(int n) { return n * QROW_TS(n) + QROW_MAXEQ; }
+constexpr int QROW_LDS = 8 * QROW_TS(8);   // per wave: the blocks of the largest system
+
 struct FitDesc {
   // ---- sizes and options
   int model;          // ModelKind
@@ -45,6 +62,8 @@ struct FitDesc {
   int nchain;         // nbeta * nseq
   uint32_t key0, key1;
   float adapt_target; // acceptance-rate threshold of the step adaptation (reference: 0.30)
+  int adapt_tie;      // rate_below (mhre_device.h): pred(adapt_target) has an even significand
+  double adapt_mid;   // rate_below: midpoint between adapt_target and the next float toward 0
   int quad_n, quad_neq;  // MODEL_QUADSYS sizes
   int record;            // 1: write pout/tout/chi2 in the sampling stage
   int hwords;            // words per acceptance history: ceil(nadj / 32)
@@ -74,6 +93,8 @@ struct FitDesc {
                            // and round naming the same visibility; null elsewhere
   int nround;              // rounds per batch (gradient only)
   const float* qcoef;      // MODEL_QUADSYS packed coefficients
+  const float* qrow;       // MODEL_QUADSYS single-coordinate update rows (QROW_* layout),
+                           // null where the step evaluates every equation in full
   const float* coor; const float* dat; const float* std2r;
   const int* icoor; const int* idat;
   const int* pdescr; const int* ivar; const int* invar;

@@ -124,7 +124,7 @@ __device__ unsigned user_one_chain(const FitDesc& f, int ch, int it, const Trace
     if (burn) {
       const int cnt = f.n_cnt[pk];
       const int a = cnt % f.nadj;   // oldest history entry, replaced by this outcome
-      uint32_t* hw = f.hbits + ((size_t)(a >> 5) * P + k) * C + ch;
+      uint32_t* hw = f.hbits + ((size_t)ch * f.hwords + (a >> 5)) * P + k;
       const uint32_t word = *hw, bit = 1u << (a & 31);
       const int hold = (word & bit) ? 1 : 0;
       *hw = acc ? (word | bit) : (word & ~bit);

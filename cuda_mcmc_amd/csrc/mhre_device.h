@@ -12,7 +12,9 @@
 //  * wave64 helpers (lane broadcast, cross-lane sums).
 #pragma once
 #include <hip/hip_runtime.h>
+#include <math.h>
 #include <stdint.h>
+#include <string.h>
 #include "approx_coeffs.h"
 
 namespace mhre {
@@ -58,6 +60,60 @@ __device__ inline float box_muller(uint32_t w1, uint32_t w2) {
   const float u1 = u01(w1), u2 = u01(w2);
   return sqrtf(-2.0f * logf(u1)) * cospif(2.0f * u2);
 }
+
+// ------------------------------------------------- step-size adaptation
+// The reference's rule (src/gpu_mcmc.cu:493, :518-522): modn = n_cnt % nadj,
+// rate = (float)sumh / (float)min(nadj, n_cnt), then the float pstp /= 1.01
+// when rate < target, else pstp *= 1.01 (both in double).  The forms below
+// give the same bits without integer or floating-point divisions.
+
+// cnt % n for 0 <= cnt, n >= 1, rn = 1.0f / n: below 2^22 the quotient
+// estimate trunc(cnt * rn) is within one of floor(cnt / n) (relative error
+// below 2^-23), so one correction each way is exact.
+__host__ __device__ inline int mod_small(int cnt, int n, float rn) {
+  if (cnt >= (1 << 22)) return cnt % n;
+  int a = cnt - (int)((float)cnt * rn) * n;
+  a += a < 0 ? n : 0;
+  a -= a >= n ? n : 0;
+  return a;
+}
+
+// (float)sh / (float)den < T for a float target T > 0, given the midpoint
+// mid = (pred(T) + T) / 2 (pred: the next float toward zero) and
+// tie_below = 1 when pred(T) has an even significand: the correctly rounded
+// quotient lies below T exactly when sh / den lies below mid, or on it when
+// the tie goes to pred(T).  mid * den is exact in double (25 significant bits
+// times den < 2^28).  den = 0: sh / 0 is -inf (below), +inf or NaN (not).
+__host__ __device__ inline bool rate_below(int sh, int den, double mid, int tie_below) {
+  if (den <= 0) return sh < 0;
+  const double md = mid * (double)den, s = (double)sh;
+  return s < md || (s == md && tie_below != 0);
+}
+
+// mid and tie_below of rate_below for the target T (a float above zero).
+inline void rate_threshold(float T, double* mid, int* tie_below) {
+  const float pred = nextafterf(T, 0.0f);
+  uint32_t pb;
+  memcpy(&pb, &pred, sizeof pb);
+  *mid = 0.5 * ((double)pred + (double)T);
+  *tie_below = (pb & 1u) == 0 ? 1 : 0;
+}
+
+// (float)((double)sd / 1.01): q0 = sd * RN(1 / 1.01) corrected once by its
+// FMA residual, the correctly rounded double quotient (Markstein); zeros and
+// infinities, whose residual is not finite or loses the sign, take q0, which
+// is exact for them.  Identical to the division for every float sd
+// (tests/step_adapt_check.hip checks all 2^32 bit patterns).
+__host__ __device__ inline float step_shrink(float sd) {
+#pragma clang fp contract(off)
+  constexpr double k = 1.01, rk = 1.0 / 1.01;
+  const double x = (double)sd;
+  const double q0 = x * rk;
+  const double q1 = fma(fma(-q0, k, x), rk, q0);
+  return (float)((x == 0.0 || __builtin_isinf(x)) ? q0 : q1);
+}
+
+__host__ __device__ inline float step_grow(float sd) { return (float)((double)sd * 1.01); }
 
 // Angle wrap into [-pi, pi] in float64 (shared bit-for-bit with the oracle).
 __host__ __device__ inline float wrap_angle(float p) {

@@ -219,11 +219,12 @@ class Session:
         """How fit i is run: its step kernel ("sgra9", "sgra13", "user",
         "quad"), whether its replica exchange runs inside a step launch,
         whether its shift phases skip the reduction before the hardware sine,
-        and the session's sequence groups."""
-        info = (C.c_int * 4)()
-        _lib.check(self.L, self.L.mhre_session_fit_info(self.h, int(i), info, 4), "mhre_session_fit_info")
+        the session's sequence groups, and whether its quadratic-system steps
+        update the residuals from row tables (one coordinate per proposal)."""
+        info = (C.c_int * 5)()
+        _lib.check(self.L, self.L.mhre_session_fit_info(self.h, int(i), info, 5), "mhre_session_fit_info")
         return dict(kernel=("sgra9", "sgra13", "user", "quad")[info[0]], exchange_in_step=bool(info[1]),
-                    hw_phase=bool(info[2]), groups=int(info[3]))
+                    hw_phase=bool(info[2]), groups=int(info[3]), quad_rows=bool(info[4]))
 
     def close(self):
         if self.h:

@@ -191,7 +191,10 @@ int mhre_session_stats(void *session, MhreStats *stats);
  * info[1] = 1 if its replica exchange runs inside a step launch (no exchange
  * launch per iteration), info[2] = 1 if its ring+Gaussian shift phases skip
  * the reduction before the hardware sine (bounded by 64 revolutions),
- * info[3] = the session's sequence groups.  ninfo >= 4. */
+ * info[3] = the session's sequence groups, and with ninfo >= 5 info[4] = 1 if
+ * its quadratic-system steps update each equation's residual from a row
+ * table when a proposal moves one coordinate, instead of evaluating every
+ * equation in full.  ninfo >= 4. */
 int mhre_session_fit_info(const void *session, int fit, int *info, int ninfo);
 int mhre_session_destroy(void *session);
 /* create + run(nburn + niter) + download + destroy. */
