@@ -93,3 +93,33 @@ def test_lockstep_rows_form(neq, n, nbeta):
     compared, counts = _lockstep(fit, f, 7, tie_rel=3e-5, edge_ulps=4, pcur_rtol=1e-6, pcur_step=1e-5)
     assert compared >= 7 * 32 * 0.5
     assert counts["ties"] <= 0.01 * counts["decisions"], counts
+
+
+def test_lockstep_rows_form_fixed_and_angle_coordinates():
+    # pdescr 0 holds coordinate 1 at its ptotal value (nprm 3 of nptot 4,
+    # ivar/invar maps: the update row is the moved coordinate's, t = ivar[k]),
+    # pdescr 2 makes coordinate 3 an angle (wrapped into [-pi, pi])
+    from cuda_mcmc_amd.engine import Fit
+    ds = synthdata.quadsys_dataset(nsys=1, neq=5, n=4, seed=17)
+    Q = ds["Q"][0].astype(np.float32)
+    b = ds["b"][0].astype(np.float32)
+    coor = np.concatenate([Q.ravel(), b.ravel()]).astype(np.float32)
+    dat = (-ds["c"][0]).astype(np.float32)
+    pdescr = np.array([1, 0, 1, 2], np.int32)
+    pmin = np.array([-5, -5, -5, -np.pi], np.float32)
+    pmax = np.array([5, 5, 5, np.pi], np.float32)
+    ptotal = np.array([0.0, 0.7, 0.0, 0.0], np.float32)
+
+    def make():
+        return Fit(pdescr, pmin, pmax, coor, dat, np.ones(5, np.float32), nbeta=16, nseq=32, nburn=4, niter=3,
+                   model="quadsys", ptotal=ptotal, seed=91, record=True)
+    fit = make()
+    assert fit.nprm == 3 and fit.nptot == 4
+    with Session([make()]) as s:
+        assert s.fit_info(0)["quad_rows"]
+    f = lambda P: generic_chi2("quadsys", P, fit.coor, fit.dat, fit.std2r)
+    compared, counts = _lockstep(fit, f, 7, tie_rel=3e-5, edge_ulps=4, pcur_rtol=1e-6, pcur_step=1e-5)
+    assert compared >= 7 * 32 * 0.5
+    assert counts["ties"] <= 0.01 * counts["decisions"], counts
+    assert np.all(fit.pcur[:, :, 1] == np.float32(0.7))
+    assert np.all(np.abs(fit.pcur[:, :, 3]) <= np.float32(np.pi))
