@@ -144,6 +144,24 @@ def test_resume_in_a_second_session(split):
     _same(ref, f)
 
 
+@pytest.mark.parametrize("split", [4, 13])
+def test_resume_quadratic_system(split):
+    # the same for a quadratic system on the row-table step: its step sizes,
+    # acceptance counts and histories (staged in LDS and chain-major on the
+    # device) go through download and upload in the reference layout
+    ref = _run([_quad(8, 77, nb=16, ns=12)])[0]
+    f = _quad(8, 77, nb=16, ns=12)
+    with Session([f]) as s:
+        assert s.fit_info(0)["quad_rows"]
+        s.run(split)
+        s.download()
+    f.it0, f.init_chains = split, False
+    with Session([f]) as s:
+        s.run(f.nburn + f.niter - split)
+        s.download()
+    _same(ref, f)
+
+
 @pytest.mark.parametrize("streams", [2, 3, 8])
 def test_sequence_groups_on_streams_bit_identical(streams):
     # sequences cut into groups that iterate on their own streams (with their
