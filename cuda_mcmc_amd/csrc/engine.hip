@@ -813,6 +813,8 @@ __device__ unsigned quad_substeps(const FitDesc& f, int ch, int it, const TraceB
   }
   float res[QROW_MAXEQ], resn[QROW_MAXEQ];
   if (ROWS) QuadSys<NQ>::residuals(qc, wave_const(f.qrow) + NQ * QROW_TS(NQ), neq, x, res);
+  // CACHE: bit k = a proposal of parameter k was accepted
+  uint32_t chg = 0;
   int flag = 0, tlast = 0;
   float told = 0.0f;
   unsigned nev = 0;
@@ -870,8 +872,12 @@ __device__ unsigned quad_substeps(const FitDesc& f, int ch, int it, const TraceB
       if ((double)(0.69314718055994531f * __builtin_amdgcn_logf(u01(r.w))) <= xx) {
         acc = 1;
         chi2c = (float)c2;
-        if (CACHE) sa[k * 64 + ln] += 1;
-        else n_acpt[pk] += 1;
+        if (CACHE) {
+          sa[k * 64 + ln] += 1;
+          chg |= 1u << k;
+        } else {
+          n_acpt[pk] += 1;
+        }
         if (ROWS) {
 #pragma unroll
           for (int i = 0; i < NQ; ++i) x[i] = (i == t) ? pn : x[i];
@@ -889,7 +895,9 @@ __device__ unsigned quad_substeps(const FitDesc& f, int ch, int it, const TraceB
     if (burn) {
       const uint32_t bit = 1u << (ha & 31);
       const int hold = (hword & bit) ? 1 : 0;
-      hbits[(ch * hwn + (ha >> 5)) * P + k] = acc ? (hword | bit) : (hword & ~bit);
+      // stored only when the entry changes: the lanes of a wave touch one
+      // history line each, so every lane left out saves a line
+      if (acc != hold) hbits[(ch * hwn + (ha >> 5)) * P + k] = hword ^ bit;
       const int sh = sh0 + acc - hold;
       const int den = cnt < nadj ? cnt : nadj;
       const float nsd = rate_below(sh, den, f.adapt_mid, f.adapt_tie) ? step_shrink(sd) : step_grow(sd);
@@ -912,21 +920,18 @@ __device__ unsigned quad_substeps(const FitDesc& f, int ch, int it, const TraceB
       gptr(tr.chi2)[ti] = inprior ? (float)c2 : 0.0f;
     }
   }
-  if (CACHE && burn) {
-#pragma unroll
-    for (int i = 0; i < NQ; ++i) {
-      if (i < P) {
-        n_cnt[(size_t)i * C + ch] = cnt_c[i];
-        sumh[(size_t)i * C + ch] = sh_c[i];
-      }
-    }
-  }
+  // write back; acceptance counts only where a proposal was accepted (a
+  // lane left out touches no line)
   if (CACHE) {
 #pragma unroll
     for (int i = 0; i < NQ; ++i) {
       if (i < P) {
-        n_acpt[i * C + ch] = sa[i * 64 + ln];
-        if (burn) pstp[i * C + ch] = sp[i * 64 + ln];
+        if (burn) {
+          n_cnt[i * C + ch] = cnt_c[i];
+          sumh[i * C + ch] = sh_c[i];
+          pstp[i * C + ch] = sp[i * 64 + ln];
+        }
+        if ((chg >> i) & 1u) n_acpt[i * C + ch] = sa[i * 64 + ln];
       }
     }
   }
