@@ -1189,8 +1189,11 @@ __device__ void iter_quad_wave(const FitDesc& f, int iseq0, int it, const TraceB
   const unsigned nev = slot ? quad_substeps<NQ, true, ROWS>(f, ch, it, tr, x, c2, qlds, stage) : 0u;
   wave_count(f, nev, slot ? (unsigned)f.nprm : 0u);
 
-  // exchange: as exchange_wave, with the partner of slot k at lane k * G + g
-  int perm = sl;
+  // exchange: as exchange_wave, with the partner of slot k at lane k * G + g.
+  // Each lane follows where its starting state goes (pos: no shuffle per
+  // attempt); afterwards one forward permute gives every slot the slot its
+  // state started in (perm), and the labels follow from it.
+  int pos = sl;
   int nex = 0;
   if (B > 1) {
     int k_l = 0;
@@ -1207,19 +1210,22 @@ __device__ void iter_quad_wave(const FitDesc& f, int iseq0, int it, const TraceB
       const float ck = __shfl(c2, lk), ck1 = __shfl(c2, lk1);
       const double xx = __shfl(hb_l, lk) * ((double)ck - (double)ck1);
       const bool acc = __shfl(lu_l, (a << lg) + g) <= xx;
-      const int tk = __shfl(tc, lk), tk1 = __shfl(tc, lk1);
-      const int pk = __shfl(perm, lk), pk1 = __shfl(perm, lk1);
       if (acc) {
-        if (sl == k) { c2 = ck1; tc = tk1; perm = pk1; if (!burn) ++nex; }
-        if (sl == k + 1) { c2 = ck; tc = tk; perm = pk; }
+        if (sl == k) { c2 = ck1; if (!burn) ++nex; }
+        if (sl == k + 1) c2 = ck;
+        pos = pos == k ? k + 1 : (pos == k + 1 ? k : pos);
       }
       if (tr.ex && sl == 0 && seq_ok && it >= tr.it0 && it < tr.it0 + tr.nit)
         tr.ex[(size_t)f.tr_ex + ((size_t)(it - tr.it0) * B + a) * S + iseq] = k | ((acc ? 1 : 0) << 16);
     }
   }
+  // pos is a permutation of each sequence's slots (lanes past the ladder keep
+  // their own), so the forward permute writes every lane once
+  const int perm = __builtin_amdgcn_ds_permute(((pos << lg) + g) << 2, sl);
+  const int src = (perm << lg) + g;
+  tc = __shfl(tc, src);
   // slot lane receives the row that started in slot perm
   float row[NQ];
-  const int src = (perm << lg) + g;
 #pragma unroll
   for (int i = 0; i < NQ; ++i) row[i] = __shfl(x[i], src);
   if (slot) {
