@@ -545,15 +545,17 @@ __device__ __attribute__((always_inline)) void step_one_chain(const FitDesc& f, 
     tlast = t;
     if (burn && lane == k) {
       // history entry cnt % nadj: the oldest one, replaced by this outcome
-      const uint32_t bit = 1u << ((cnt % f.nadj) & 31);
+      // (the division-free forms of mhre_device.h give the reference's bits)
+      const int nadj = f.nadj;
+      const float rnadj = 1.0f / (float)nadj;
+      const uint32_t bit = 1u << (mod_small(cnt, nadj, rnadj) & 31);
       const int hold = (hword & bit) ? 1 : 0;
       hword = acc ? (hword | bit) : (hword & ~bit);
       sh += acc - hold;
-      const int den = cnt < f.nadj ? cnt : f.nadj;
-      const float rate = (float)sh / (float)den;
-      stp = rate < f.adapt_target ? (float)((double)stp / 1.01) : (float)((double)stp * 1.01);
+      const int den = cnt < nadj ? cnt : nadj;
+      stp = rate_below(sh, den, f.adapt_mid, f.adapt_tie) ? step_shrink(stp) : step_grow(stp);
       ++cnt;
-      const int w2 = (cnt % f.nadj) >> 5;
+      const int w2 = mod_small(cnt, nadj, rnadj) >> 5;
       if (w2 != hw_idx) {
         f.hbits[(ch * f.hwords + hw_idx) * P + lane] = hword;
         hw_idx = w2;
